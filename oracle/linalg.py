@@ -1,0 +1,125 @@
+"""Small dense linear algebra of the sweep (oracle).
+
+* absorb_matrix_at  -- NDArrayData.absorbMatrixAt (carcassonne/data/__init__.py:148-150)
+* normalize_axis    -- NDArrayData.normalizeAxis (data/__init__.py:263-301)
+* unitize           -- polar factor U V^H (carcassonne/utils.py:879-882)
+* new_enlargener    -- NDArrayData.newEnlargener (data/__init__.py:43-50)
+* product_compressor-- computeProductCompressor (carcassonne/compression.py:11-46)
+
+Pinned by: normalize_axis -- isometry and inverse identities
+(reference test_data.py:98-133 restated in tests/test_oracle_linalg.py);
+product_compressor -- exact reconstruction when the target rank suffices
+(reference test_compression.py:9-30, test_system.py:40-81).  The raw compressor
+after four non-converged ALS sweeps is "parity unpinned" (the reference starts
+from an unseeded random isometry and uses SciPy GMRES defaults).
+"""
+import numpy as np
+from scipy.linalg import svd
+
+
+def join(a, *groups):
+    """NDArrayData.join (data/__init__.py:247-256): transpose by the flattened groups,
+    then merge each group into one axis."""
+    groups = [[g] if isinstance(g, (int, np.integer)) else list(g) for g in groups]
+    t = a.transpose([i for g in groups for i in g])
+    shape, k = [], 0
+    for g in groups:
+        shape.append(int(np.prod(t.shape[k:k + len(g)], dtype=np.int64)))
+        k += len(g)
+    return t.reshape(shape)
+
+
+def absorb_matrix_at(a, axis, m):
+    """new[..., i, ...] = sum_j m[i, j] a[..., j, ...]."""
+    return np.moveaxis(np.tensordot(m, a, (1, axis)), 0, axis)
+
+
+def unitize(m):
+    u, _, vh = svd(m, full_matrices=False)
+    return u @ vh
+
+
+def normalize_axis(a, axis, sqrt_svals=False, dont_recip_under=1e-14):
+    """Returns (isometry, normalizer, denormalizer), or (normalizer, denormalizer)
+    when sqrt_svals, exactly as data/__init__.py:263-301."""
+    if a.shape[axis] == 1:
+        n = np.linalg.norm(a)
+        if sqrt_svals:
+            n = np.sqrt(n)
+            return np.array([[1 / n]]), np.array([[n]])
+        return a / n, np.array([[1 / n]]), np.array([[n]])
+    others = [i for i in range(a.ndim) if i != axis]
+    M = join(a, others, axis)
+    if M.shape[0] < M.shape[1]:
+        raise ValueError("the total number of degrees of freedom in all other axes ({}) are not enough "
+                         "to normalize axis ({}) with dimension ({})".format(M.shape[0], axis, M.shape[1]))
+    U, S, V = svd(M, full_matrices=False)
+    U = U @ V
+    S = S.reshape(-1, 1)
+    SI = S.copy()
+    if dont_recip_under:
+        nz = np.abs(SI) > dont_recip_under
+        SI[nz] = 1.0 / SI[nz]
+    else:
+        SI = 1.0 / SI
+    if sqrt_svals:
+        S, SI = np.sqrt(S), np.sqrt(SI)
+        return (V * SI).conj(), V * S
+    shape = [a.shape[i] for i in others] + [U.shape[1]]
+    iso = np.moveaxis(U.reshape(shape), -1, axis)
+    return iso, V.T @ (V * SI).conj(), V.T.conj() @ (V * S)
+
+
+def normalize_axis_and_denormalize(a, axis_to_norm, axis_to_denorm, other=None):
+    """normalizeAxisAndDenormalize (data/__init__.py:302-310)."""
+    if other is None:
+        other = a
+    if a.shape[axis_to_norm] != other.shape[axis_to_denorm]:
+        raise ValueError("Normalized axis and denormalized axis have different sizes")
+    iso, _, den = normalize_axis(a, axis_to_norm)
+    return iso, absorb_matrix_at(other, axis_to_denorm, den)
+
+
+def new_enlargener(old, new, rng):
+    """newEnlargener (data/__init__.py:43-50): identity when equal, else the Q factor of a
+    random new x old matrix (and its conjugate)."""
+    if new == old:
+        e = np.identity(new)
+        return e, e
+    q, _ = np.linalg.qr(rng.standard_normal((new, old)))
+    return q, q.conj()
+
+
+def compressor_matrix(Lt, U, Rt):
+    """The generated contractor of compression.py:12-25 evaluated on (L, U*, U^H, U^T, R):
+    joins L2<->U*0, U^T1<->R1 (via U^H/U^T), U*1<->U^T0, L3<->R2; result
+    ((L0,R3),(L1,U^H0)) -- a matrix (l*r) x (old*new) acting on vec(x[old,new])."""
+    Uc = U.conj()
+    # A[(l,r),(o,n)] = sum L[l,o,p,q] Uc[p,m] UH[n',?]...  written from the join list:
+    #  tensors: 0=L(l,o1,o2,w) 1=U*(o2,m) 2=U^H(n,o3) 3=U^T(m,o4) 4=R(o3,o4,w,r)
+    #  joins: L2-U*0, U^H1-R0, U^T1-R1, U*1-U^T0, L3-R2 ; groups ((L0,R3),(L1,U^H0))
+    UH = U.conj().T
+    UT = U.T
+    A = np.einsum("lapw,pm,nb,mc,bcwr->lran", Lt, Uc, UH, UT, Rt)
+    l, r, a, n = A.shape
+    return A.reshape(l * r, a * n)
+
+
+def product_compressor(Lt, Rt, new_dimension, rng, sweeps=4):
+    """computeProductCompressor (compression.py:26-45).  The normal equations
+    A^H A x = A^H b are solved exactly (lstsq) where the reference calls GMRES with
+    SciPy defaults; both target the same least-squares x.  Returns U^T (new x old)."""
+    if Lt.shape[1] != Lt.shape[2]:
+        raise ValueError("left inward dimensions do not match (given " + str(Lt.shape) + ")")
+    if Rt.shape[0] != Rt.shape[1]:
+        raise ValueError("right inward dimensions do not match (given " + str(Rt.shape) + ")")
+    if Lt.shape[1] != Rt.shape[1]:
+        raise ValueError("left and right shapes are incompatible")
+    old = Lt.shape[1]
+    b = np.tensordot(Lt, Rt, ((1, 2, 3), (0, 1, 2))).ravel()
+    U = unitize(rng.standard_normal((old, new_dimension)))
+    for _ in range(sweeps):
+        A = compressor_matrix(Lt, U, Rt)
+        x = np.linalg.lstsq(A, b, rcond=None)[0]
+        U = unitize(x.reshape(old, new_dimension))
+    return U.T
