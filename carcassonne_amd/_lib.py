@@ -1,0 +1,98 @@
+"""ctypes binding of libcarcassonne_hip.so (the C ABI declared in include/carcassonne_hip.h).
+
+The product path has no CPU fallback: if the library is missing or fails to
+load, every entry point raises LibraryMissingError.
+"""
+import ctypes
+import os
+
+from .errors import HipError, LibraryMissingError
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcarcassonne_hip.so")
+
+CC_OK, CC_ERR_RANK, CC_ERR_DIM, CC_ERR_NOT_CONVERGED = 0, -1, -2, -3
+CC_ERR_HIP, CC_ERR_OOM, CC_ERR_ARG, CC_ERR_UNSUPPORTED = -4, -5, -6, -7
+CC_F64, CC_F32, CC_F16, CC_BF16 = 0, 1, 2, 3
+
+_vp, _i64, _i32, _f64 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
+_pi64, _pvp = ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_void_p)
+
+# name -> argtypes (all return int status)
+SIGNATURES = {
+    "cc_version": [],
+    "cc_permute": [_vp, _i32, _vp, _vp, _i32, _pi64, _pi64],
+    "cc_gemm_f64": [_vp, _i64, _i64, _i64, _i64, _f64, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _f64,
+                    _vp, _i64, _i64, _i64],
+    "cc_axpby_f64": [_vp, _i64, _f64, _vp, _f64, _vp],
+    "cc_fill_f64": [_vp, _i64, _f64, _vp],
+    "cc_set_identity_f64": [_vp, _i64, _vp],
+    "cc_dot_f64": [_vp, _i64, _vp, _vp, _vp],
+    "cc_count_nonfinite_f64": [_vp, _i64, _vp, _vp],
+    "cc_cast": [_vp, _i64, _i32, _vp, _i32, _vp],
+    "cc_binary_f64": [_vp, _i32, _i64, _vp, _vp, _vp],
+    "cc_unary_f64": [_vp, _i32, _i64, _vp, _f64, _vp],
+    "cc_scale_rows_cols_f64": [_vp, _i32, _i64, _i64, _vp, _vp, _vp],
+    "cc_svd_work_size": [_i64, _i64, _i64, _pi64],
+    "cc_svd_f64": [_vp, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
+    "cc_stage3_matvec_generic_work_size": [_i64, _i64, _i64, _i64, _pi64],
+    "cc_stage3_matvec_generic_f64": [_vp, _i32, _pvp, _pvp, _pvp, _i64, _i64, _i64, _i64, _vp, _vp, _vp],
+}
+
+_lib = None
+_load_error = None
+
+
+def load():
+    global _lib, _load_error
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        _load_error = "native library not built: " + LIB_PATH + " (run __graft_entry__.build())"
+        raise LibraryMissingError(_load_error)
+    try:
+        lib = ctypes.CDLL(LIB_PATH)
+    except OSError as e:
+        _load_error = str(e)
+        raise LibraryMissingError("cannot load {}: {}".format(LIB_PATH, e))
+    lib.cc_last_error.restype = ctypes.c_char_p
+    lib.cc_last_error.argtypes = []
+    for name, args in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = ctypes.c_int
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return list(SIGNATURES) + ["cc_last_error"]
+
+
+def check(rc, what=""):
+    if rc == CC_OK:
+        return
+    msg = load().cc_last_error().decode(errors="replace")
+    from .errors import RelaxFailed
+    if rc == CC_ERR_ARG or rc == CC_ERR_DIM or rc == CC_ERR_RANK:
+        raise ValueError("{}: {}".format(what, msg))
+    if rc == CC_ERR_NOT_CONVERGED:
+        raise RelaxFailed(float("nan"), float("nan"))
+    if rc == CC_ERR_UNSUPPORTED:
+        raise NotImplementedError("{}: {}".format(what, msg))
+    if rc == CC_ERR_OOM:
+        raise MemoryError("{}: {}".format(what, msg))
+    raise HipError("{} failed ({}): {}".format(what, rc, msg))
+
+
+def call(name, *args):
+    lib = load()
+    check(getattr(lib, name)(*args), name)
+
+
+def i64_array(values):
+    values = [int(v) for v in values]
+    return (ctypes.c_int64 * max(1, len(values)))(*values)
+
+
+def ptr_array(ptrs):
+    return (ctypes.c_void_p * max(1, len(ptrs)))(*[ctypes.c_void_p(p) if p else None for p in ptrs])

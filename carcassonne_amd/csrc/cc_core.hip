@@ -1,0 +1,233 @@
+// Data-protocol primitives: permute, elementwise, fixed-order reductions, casts.
+#include "cc_common.h"
+
+#include <cstring>
+#include <string>
+
+static thread_local std::string g_last_error;
+void cc_set_last_error(const char* msg) { g_last_error = msg ? msg : ""; }
+
+extern "C" int cc_version(void) { return 1; }
+extern "C" const char* cc_last_error(void) { return g_last_error.c_str(); }
+
+// ---------------------------------------------------------------- permute
+#define CC_MAX_RANK 16
+struct PermDesc {
+  int rank;
+  int64_t shape[CC_MAX_RANK];
+  int64_t stride[CC_MAX_RANK];
+};
+
+template <typename T>
+__global__ void permute_kernel(const T* __restrict__ src, T* __restrict__ dst, int64_t total, PermDesc d) {
+  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total;
+       o += (int64_t)gridDim.x * blockDim.x) {
+    int64_t rem = o, off = 0;
+    for (int k = d.rank - 1; k >= 0; --k) {
+      int64_t i = rem % d.shape[k];
+      rem /= d.shape[k];
+      off += i * d.stride[k];
+    }
+    dst[o] = src[off];
+  }
+}
+
+static int dtype_size(int dt) {
+  switch (dt) {
+    case CC_F64: return 8;
+    case CC_F32: return 4;
+    case CC_F16: return 2;
+    case CC_BF16: return 2;
+  }
+  return 0;
+}
+
+extern "C" int cc_permute(void* stream, int dtype, const void* src, void* dst, int rank,
+                          const int64_t* shape, const int64_t* src_strides) {
+  if (rank < 0 || rank > CC_MAX_RANK) { cc_set_last_error("cc_permute: rank out of range"); return CC_ERR_ARG; }
+  PermDesc d;
+  d.rank = rank;
+  int64_t total = 1;
+  for (int k = 0; k < rank; ++k) {
+    if (shape[k] < 0) { cc_set_last_error("cc_permute: negative extent"); return CC_ERR_ARG; }
+    d.shape[k] = shape[k];
+    d.stride[k] = src_strides[k];
+    total *= shape[k];
+  }
+  if (total == 0) return CC_OK;
+  unsigned blocks = cc_blocks(total, 256);
+  switch (dtype_size(dtype)) {
+    case 8: permute_kernel<uint64_t><<<blocks, 256, 0, cc_stream(stream)>>>((const uint64_t*)src, (uint64_t*)dst, total, d); break;
+    case 4: permute_kernel<uint32_t><<<blocks, 256, 0, cc_stream(stream)>>>((const uint32_t*)src, (uint32_t*)dst, total, d); break;
+    case 2: permute_kernel<uint16_t><<<blocks, 256, 0, cc_stream(stream)>>>((const uint16_t*)src, (uint16_t*)dst, total, d); break;
+    default: cc_set_last_error("cc_permute: bad dtype"); return CC_ERR_ARG;
+  }
+  CC_LAUNCH_CHECK();
+  return CC_OK;
+}
+
+// ---------------------------------------------------------------- elementwise
+__global__ void axpby_kernel(int64_t n, double a, const double* __restrict__ x, double b, double* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (b == 0.0 ? 0.0 : b * y[i]) + a * x[i];
+}
+__global__ void fill_kernel(int64_t n, double v, double* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) y[i] = v;
+}
+__global__ void set_identity_kernel(int64_t n, double* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) y[i * (n + 1)] = 1.0;
+}
+
+extern "C" int cc_axpby_f64(void* stream, int64_t n, double alpha, const double* x, double beta, double* y) {
+  if (n <= 0) return CC_OK;
+  axpby_kernel<<<cc_blocks(n, 256), 256, 0, cc_stream(stream)>>>(n, alpha, x, beta, y);
+  CC_LAUNCH_CHECK();
+  return CC_OK;
+}
+extern "C" int cc_fill_f64(void* stream, int64_t n, double value, double* y) {
+  if (n <= 0) return CC_OK;
+  fill_kernel<<<cc_blocks(n, 256), 256, 0, cc_stream(stream)>>>(n, value, y);
+  CC_LAUNCH_CHECK();
+  return CC_OK;
+}
+extern "C" int cc_set_identity_f64(void* stream, int64_t n, double* y) {
+  if (n <= 0) return CC_OK;
+  set_identity_kernel<<<cc_blocks(n, 256), 256, 0, cc_stream(stream)>>>(n, y);
+  CC_LAUNCH_CHECK();
+  return CC_OK;
+}
+
+// ---------------------------------------------------------------- reductions
+// One block of 1024 threads: each thread sums a fixed stride class, then a fixed
+// tree.  The summation order depends only on n, never on scheduling.
+__global__ void __launch_bounds__(1024) dot_kernel(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+                                                   double* __restrict__ out) {
+  __shared__ double red[1024];
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) acc += x[i] * y[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+__global__ void __launch_bounds__(1024) count_nonfinite_kernel(int64_t n, const double* __restrict__ x, int64_t* __restrict__ out) {
+  __shared__ int64_t red[1024];
+  int64_t acc = 0;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) acc += isfinite(x[i]) ? 0 : 1;
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+extern "C" int cc_dot_f64(void* stream, int64_t n, const double* x, const double* y, double* out) {
+  dot_kernel<<<1, 1024, 0, cc_stream(stream)>>>(n < 0 ? 0 : n, x, y, out);
+  CC_LAUNCH_CHECK();
+  return CC_OK;
+}
+extern "C" int cc_count_nonfinite_f64(void* stream, int64_t n, const double* x, int64_t* out) {
+  count_nonfinite_kernel<<<1, 1024, 0, cc_stream(stream)>>>(n < 0 ? 0 : n, x, out);
+  CC_LAUNCH_CHECK();
+  return CC_OK;
+}
+
+// ---------------------------------------------------------------- casts
+template <typename S>
+__device__ inline double load_as_f64(const S* p, int64_t i) { return (double)p[i]; }
+
+template <typename S, typename D>
+__global__ void cast_kernel(int64_t n, const S* __restrict__ s, D* __restrict__ d) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = (D)(float)(double)s[i];
+}
+template <typename S>
+__global__ void cast_to_f64_kernel(int64_t n, const S* __restrict__ s, double* __restrict__ d) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = (double)(float)s[i];
+}
+__global__ void cast_f64_f32_kernel(int64_t n, const double* __restrict__ s, float* __restrict__ d) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) d[i] = (float)s[i];
+}
+__global__ void cast_f32_f64_kernel(int64_t n, const float* __restrict__ s, double* __restrict__ d) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) d[i] = (double)s[i];
+}
+// f64 -> f16/bf16 rounds once (round-to-nearest-even from the double value)
+__global__ void cast_f64_f16_kernel(int64_t n, const double* __restrict__ s, _Float16* __restrict__ d) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) d[i] = (_Float16)s[i];
+}
+__global__ void cast_f64_bf16_kernel(int64_t n, const double* __restrict__ s, __bf16* __restrict__ d) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) d[i] = (__bf16)s[i];
+}
+
+extern "C" int cc_cast(void* stream, int64_t n, int src_dtype, const void* src, int dst_dtype, void* dst) {
+  if (n <= 0) return CC_OK;
+  hipStream_t st = cc_stream(stream);
+  unsigned b = cc_blocks(n, 256);
+  if (src_dtype == CC_F64 && dst_dtype == CC_F32) cast_f64_f32_kernel<<<b, 256, 0, st>>>(n, (const double*)src, (float*)dst);
+  else if (src_dtype == CC_F32 && dst_dtype == CC_F64) cast_f32_f64_kernel<<<b, 256, 0, st>>>(n, (const float*)src, (double*)dst);
+  else if (src_dtype == CC_F64 && dst_dtype == CC_F16) cast_f64_f16_kernel<<<b, 256, 0, st>>>(n, (const double*)src, (_Float16*)dst);
+  else if (src_dtype == CC_F64 && dst_dtype == CC_BF16) cast_f64_bf16_kernel<<<b, 256, 0, st>>>(n, (const double*)src, (__bf16*)dst);
+  else if (src_dtype == CC_F16 && dst_dtype == CC_F64) cast_to_f64_kernel<_Float16><<<b, 256, 0, st>>>(n, (const _Float16*)src, (double*)dst);
+  else if (src_dtype == CC_BF16 && dst_dtype == CC_F64) cast_to_f64_kernel<__bf16><<<b, 256, 0, st>>>(n, (const __bf16*)src, (double*)dst);
+  else { cc_set_last_error("cc_cast: unsupported dtype pair"); return CC_ERR_UNSUPPORTED; }
+  CC_LAUNCH_CHECK();
+  return CC_OK;
+}
+
+// ---------------------------------------------------------------- more elementwise
+// binary: out = x (op) y ; unary: out = f(x; param)
+__global__ void binary_kernel(int op, int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+                              double* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double a = x[i], b = y[i];
+    out[i] = (op == 0) ? a * b : (op == 1) ? a / b : (op == 2) ? a + b : a - b;
+  }
+}
+__global__ void unary_kernel(int op, int64_t n, const double* __restrict__ x, double p, double* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double a = x[i], r;
+    switch (op) {
+      case 0: r = sqrt(a); break;                          // sqrt
+      case 1: r = fabs(a) > p ? 1.0 / a : a; break;        // reciprocal above threshold (else unchanged)
+      case 2: r = fabs(a); break;                          // abs
+      case 3: r = a * p; break;                            // scale
+      default: r = 1.0 / a; break;                         // reciprocal
+    }
+    out[i] = r;
+  }
+}
+// out[i][j] = A[i][j] * s[i] (rows) or * s[j] (cols)
+__global__ void scale_rc_kernel(int cols, int64_t m, int64_t n, const double* __restrict__ A, const double* __restrict__ s,
+                                double* __restrict__ out) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m * n; e += (int64_t)gridDim.x * blockDim.x)
+    out[e] = A[e] * (cols ? s[e % n] : s[e / n]);
+}
+
+extern "C" int cc_binary_f64(void* stream, int op, int64_t n, const double* x, const double* y, double* out) {
+  if (n <= 0) return CC_OK;
+  if (op < 0 || op > 3) { cc_set_last_error("cc_binary_f64: bad op"); return CC_ERR_ARG; }
+  binary_kernel<<<cc_blocks(n, 256), 256, 0, cc_stream(stream)>>>(op, n, x, y, out);
+  CC_LAUNCH_CHECK();
+  return CC_OK;
+}
+extern "C" int cc_unary_f64(void* stream, int op, int64_t n, const double* x, double param, double* out) {
+  if (n <= 0) return CC_OK;
+  if (op < 0 || op > 4) { cc_set_last_error("cc_unary_f64: bad op"); return CC_ERR_ARG; }
+  unary_kernel<<<cc_blocks(n, 256), 256, 0, cc_stream(stream)>>>(op, n, x, param, out);
+  CC_LAUNCH_CHECK();
+  return CC_OK;
+}
+extern "C" int cc_scale_rows_cols_f64(void* stream, int cols, int64_t m, int64_t n, const double* A, const double* s,
+                                      double* out) {
+  if (m <= 0 || n <= 0) return CC_OK;
+  scale_rc_kernel<<<cc_blocks(m * n, 256), 256, 0, cc_stream(stream)>>>(cols, m, n, A, s, out);
+  CC_LAUNCH_CHECK();
+  return CC_OK;
+}

@@ -1,0 +1,136 @@
+// Thin SVD by one-sided Jacobi (Hestenes) with the round-robin parallel ordering:
+// one 1024-thread workgroup per matrix, one wave per disjoint column pair per round.
+// Columns live in global memory (`work`), so m is unbounded; n <= 256.
+#include "cc_common.h"
+
+namespace {
+constexpr int SVD_THREADS = 1024;
+constexpr int SVD_MAX_N = 256;
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__global__ void __launch_bounds__(SVD_THREADS) svd_jacobi_kernel(int64_t m, int n, const double* __restrict__ A,
+                                                                 double* __restrict__ U, double* __restrict__ S,
+                                                                 double* __restrict__ Vt, double* __restrict__ work) {
+  const int64_t b = blockIdx.x;
+  A += b * m * n;
+  U += b * m * n;
+  S += b * n;
+  Vt += b * n * n;
+  double* W = work + b * (m * n + (int64_t)n * n);  // W[j][i] = column j of A (n x m)
+  double* V = W + m * n;                             // V[j][k] = column j of V (n x n)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = SVD_THREADS / 64;
+  __shared__ int pairs[SVD_MAX_N + 1];
+  __shared__ int rotated;
+  __shared__ double snorm[SVD_MAX_N];
+  __shared__ int perm[SVD_MAX_N];
+
+  for (int64_t e = tid; e < m * n; e += SVD_THREADS) {
+    int64_t i = e / n;
+    int j = (int)(e % n);
+    W[(int64_t)j * m + i] = A[e];
+  }
+  for (int e = tid; e < n * n; e += SVD_THREADS) V[e] = (e / n == e % n) ? 1.0 : 0.0;
+  const int np = (n + 1) & ~1;  // even count; index n (when odd) is a dummy
+  for (int e = tid; e < np; e += SVD_THREADS) pairs[e] = e;
+  __syncthreads();
+
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    if (tid == 0) rotated = 0;
+    __syncthreads();
+    for (int round = 0; round < np - 1; ++round) {
+      for (int pr = wave; pr < np / 2; pr += nwaves) {
+        int p = pairs[pr], q = pairs[np - 1 - pr];
+        if (p >= n || q >= n) continue;
+        if (p > q) { int t = p; p = q; q = t; }
+        double* wp = W + (int64_t)p * m;
+        double* wq = W + (int64_t)q * m;
+        double al = 0.0, be = 0.0, ga = 0.0;
+        for (int64_t i = lane; i < m; i += 64) {
+          double x = wp[i], y = wq[i];
+          al += x * x;
+          be += y * y;
+          ga += x * y;
+        }
+        al = wave_sum(al);
+        be = wave_sum(be);
+        ga = wave_sum(ga);
+        if (ga != 0.0 && fabs(ga) > 1e-15 * sqrt(al * be)) {
+          double zeta = (be - al) / (2.0 * ga);
+          double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+          for (int64_t i = lane; i < m; i += 64) {
+            double x = wp[i], y = wq[i];
+            wp[i] = c * x - s * y;
+            wq[i] = s * x + c * y;
+          }
+          double* vp = V + (int64_t)p * n;
+          double* vq = V + (int64_t)q * n;
+          for (int k = lane; k < n; k += 64) {
+            double x = vp[k], y = vq[k];
+            vp[k] = c * x - s * y;
+            vq[k] = s * x + c * y;
+          }
+          if (lane == 0) rotated = 1;
+        }
+      }
+      __syncthreads();
+      // rotate the tournament: keep pairs[0], shift the rest by one
+      if (tid == 0) {
+        int last = pairs[np - 1];
+        for (int e = np - 1; e > 1; --e) pairs[e] = pairs[e - 1];
+        pairs[1] = last;
+      }
+      __syncthreads();
+    }
+    if (!rotated) break;
+  }
+
+  // singular values = column norms, sorted descending (stable by index)
+  for (int j = wave; j < n; j += nwaves) {
+    double s2 = 0.0;
+    for (int64_t i = lane; i < m; i += 64) { double x = W[(int64_t)j * m + i]; s2 += x * x; }
+    s2 = wave_sum(s2);
+    if (lane == 0) snorm[j] = sqrt(s2);
+  }
+  __syncthreads();
+  for (int j = tid; j < n; j += SVD_THREADS) {
+    int r = 0;
+    for (int k = 0; k < n; ++k) r += (snorm[k] > snorm[j]) || (snorm[k] == snorm[j] && k < j);
+    perm[r] = j;
+  }
+  __syncthreads();
+  for (int r = tid; r < n; r += SVD_THREADS) S[r] = snorm[perm[r]];
+  for (int64_t e = tid; e < m * n; e += SVD_THREADS) {
+    int64_t i = e / n;
+    int r = (int)(e % n);
+    int j = perm[r];
+    double s = snorm[j];
+    U[e] = s > 0.0 ? W[(int64_t)j * m + i] / s : 0.0;
+  }
+  for (int e = tid; e < n * n; e += SVD_THREADS) {
+    int r = e / n, k = e % n;
+    Vt[e] = V[(int64_t)perm[r] * n + k];
+  }
+}
+}  // namespace
+
+extern "C" int cc_svd_work_size(int64_t batch, int64_t m, int64_t n, int64_t* out) {
+  *out = batch * (m * n + n * n);
+  return CC_OK;
+}
+
+extern "C" int cc_svd_f64(void* stream, int64_t batch, int64_t m, int64_t n, const double* A, double* U, double* S,
+                          double* Vt, double* work) {
+  if (batch < 0 || m < 0 || n < 0) { cc_set_last_error("cc_svd_f64: negative size"); return CC_ERR_ARG; }
+  if (n > SVD_MAX_N) { cc_set_last_error("cc_svd_f64: more than 256 columns"); return CC_ERR_UNSUPPORTED; }
+  if (m < n) { cc_set_last_error("cc_svd_f64: needs m >= n"); return CC_ERR_ARG; }
+  if (batch == 0 || n == 0) return CC_OK;
+  svd_jacobi_kernel<<<(unsigned)batch, SVD_THREADS, 0, cc_stream(stream)>>>(m, (int)n, A, U, S, Vt, work);
+  CC_LAUNCH_CHECK();
+  return CC_OK;
+}

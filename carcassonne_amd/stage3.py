@@ -1,0 +1,103 @@
+"""The center-site effective-Hamiltonian operator of one solve.
+
+Re-implements formNormalizationStage3 / formDenseStage3 / multiplyExpectation
+(carcassonne/tensors/_2d/dense.py:115-203, carcassonne/tensors/_2d/sparse.py:129-148):
+
+    (H v)[p', q', s'] = sum_t sum_alpha sum_{p,q,s} A_t[alpha][p'][p] B_t[alpha][q'][q] O_t[s'][s] v[p, q, s]
+
+with A_t = stage2_0.join((0,1),4,5,2,3) and B_t = stage2_1.join((1,0),4,5,2,3) -- the
+alpha-major slab layout -- and O_t the center operator (identity for the
+normalization contractor).  All terms of a solve are applied in one call to the
+HIP library: the fused MFMA kernel for the instantiated slab shapes, the generic
+GEMM path for any other shape.
+"""
+import numpy as np
+
+from . import _lib
+from .data import HipData
+from .device import empty, ptr, stream
+
+
+def cost_of_term(n_alpha, P1, P, Q1, Q, d, with_op):
+    """The reference's CostTracker count (complex MACs) of one stage-3 contractor:
+    [O<->v], then A<->v over (D0,D1), then B<->T over (D2,D3,alpha)."""
+    c = (d * P * Q) * d if with_op else 0
+    c += (n_alpha * P1 * Q * d) * P
+    c += (Q1 * P1 * d) * (Q * n_alpha)
+    return int(c)
+
+
+def cost_of_matrix(s2_0_shape, s2_1_shape, d):
+    x, y, k0, k1, b0, b1 = s2_0_shape
+    _, _, k2, k3, b2, b3 = s2_1_shape
+    m = k0 * k1 * b0 * b1 * k2 * k3 * b2 * b3
+    return int(m * x * y + m * d * d)
+
+
+class Stage3Operator:
+    """terms: list of (stage2_0, stage2_1, site_operator or None) with HipData tensors."""
+
+    def __init__(self, terms, physical_dimension=None):
+        self.terms = list(terms)
+        if not self.terms:
+            raise ValueError("a stage-3 operator needs at least one term")
+        s0, s1, o = self.terms[0]
+        self.d = int(physical_dimension if physical_dimension is not None else
+                     next((op.shape[0] for (_, _, op) in self.terms if op is not None), None))
+        self.state_shape = (s0.shape[2], s0.shape[3], s1.shape[2], s1.shape[3], self.d)
+        self.n = int(np.prod(self.state_shape))
+        self.P, self.Q = s0.shape[2] * s0.shape[3], s1.shape[2] * s1.shape[3]
+        self.n_alpha = s0.shape[0] * s0.shape[1]
+        self._joined = {}
+        self._A, self._B, self._O = [], [], []
+        for (a, b, op) in self.terms:
+            if a.shape[0] * a.shape[1] != self.n_alpha or b.shape[0] * b.shape[1] != self.n_alpha:
+                raise ValueError("stage-2 tensors of one operator must share the environment bond")
+            self._A.append(self._join(a, 0))
+            self._B.append(self._join(b, 1))
+            self._O.append(op)
+        self.cost_of_multiply = sum(cost_of_term(self.n_alpha, a.shape[1] * a.shape[2], self.P, b.shape[1] * b.shape[2],
+                                                 self.Q, self.d, op is not None)
+                                    for a, b, op in zip(self._A, self._B, self._O))
+        self.cost_of_formMatrix = sum(cost_of_matrix(a.shape, b.shape, self.d) for (a, b, _) in self.terms)
+        self._work = None
+
+    def _join(self, t, which):
+        key = (id(t), which)
+        if key not in self._joined:
+            self._joined[key] = t.join((0, 1), 4, 5, 2, 3) if which == 0 else t.join((1, 0), 4, 5, 2, 3)
+        return self._joined[key]
+
+    def apply_flat(self, x, y):
+        """y = H x for flat device tensors of length n (torch float64)."""
+        if self._work is None:
+            ws = _lib.i64_array([0])
+            _lib.call("cc_stage3_matvec_generic_work_size", self.n_alpha, self.P, self.Q, self.d, ws)
+            self._work = empty((max(1, ws[0]),))
+        _lib.call("cc_stage3_matvec_generic_f64", stream(), len(self._A),
+                  _lib.ptr_array([a._t.data_ptr() for a in self._A]),
+                  _lib.ptr_array([b._t.data_ptr() for b in self._B]),
+                  _lib.ptr_array([o._t.data_ptr() if o is not None else 0 for o in self._O]),
+                  self.n_alpha, self.P, self.Q, self.d, ptr(x), ptr(y), ptr(self._work))
+
+    def apply(self, v):
+        if tuple(v.shape) != self.state_shape and v.size() != self.n:
+            raise ValueError("state shape {} does not match the operator's {}".format(v.shape, self.state_shape))
+        out = empty(self.state_shape)
+        self.apply_flat(v._t.reshape(-1), out.view(-1))
+        return HipData(out)
+
+    def form_matrix(self):
+        from .tensors._2d.dense import _form_matrix
+        eye = HipData.newIdentity(self.d)
+        m = None
+        for (a, b, op) in self.terms:
+            t = _form_matrix(a, b, op if op is not None else eye)
+            m = t if m is None else m + t
+        return m
+
+    def multiplier(self):
+        from .utils import Multiplier
+        m = Multiplier((self.n, self.n), self.apply, self.cost_of_multiply, self.form_matrix, self.cost_of_formMatrix)
+        m.operator = self
+        return m

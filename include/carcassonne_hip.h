@@ -1,0 +1,115 @@
+/*
+ * carcassonne_hip.h -- C ABI of the MI355X (gfx950) implementation of
+ * Carcassonne's variational-sweep hot path.
+ *
+ * Conventions
+ *   - Every function returns an int status (CC_OK = 0, negative on error);
+ *     cc_last_error() returns a thread-local message for the last failure.
+ *   - Pointers are device pointers unless named host_*; sizes are element
+ *     counts; tensors are dense row-major unless strides are passed.
+ *   - All work is stream-ordered on the caller's hipStream_t (passed as void*).
+ *     The caller owns every buffer; the library never frees caller memory.
+ *   - Reductions run in a fixed order, so results do not depend on scheduling.
+ *
+ * Each entry point names the reference (gcross/Carcassonne) interface whose
+ * arithmetic it carries out, as file:line in that repository.
+ */
+#ifndef CARCASSONNE_HIP_H
+#define CARCASSONNE_HIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes -- the Python layer maps them to the reference's exceptions */
+#define CC_OK 0
+#define CC_ERR_RANK (-1)          /* UnexpectedTensorRankError  carcassonne/utils.py:34-41 */
+#define CC_ERR_DIM (-2)           /* DimensionMismatchError     carcassonne/utils.py:13-23 */
+#define CC_ERR_NOT_CONVERGED (-3) /* RelaxFailed                carcassonne/utils.py:25-33 */
+#define CC_ERR_HIP (-4)           /* HIP runtime failure */
+#define CC_ERR_OOM (-5)           /* device allocation failure */
+#define CC_ERR_ARG (-6)           /* ValueError */
+#define CC_ERR_UNSUPPORTED (-7)   /* shape / dtype without a kernel */
+
+/* element types */
+#define CC_F64 0
+#define CC_F32 1
+#define CC_F16 2
+#define CC_BF16 3
+
+int cc_version(void);
+const char* cc_last_error(void);
+
+/* ---- Data protocol primitives (NDArrayData, carcassonne/data/__init__.py) ---- */
+
+/* dst = src viewed with `shape` and element strides `src_strides`, written densely
+ * (row-major).  NDArrayData.join / transpose   data/__init__.py:247-256, 347-349 */
+int cc_permute(void* stream, int dtype, const void* src, void* dst, int rank,
+               const int64_t* shape, const int64_t* src_strides);
+
+/* C[b] = alpha * A[b] @ B[b] + beta * C[b] with arbitrary element strides
+ * (A: b,m,k  B: b,k,n  C: b,m,n).  NDArrayData.contractWith (numpy.tensordot)
+ * data/__init__.py:157-159; every generated contractor of utils.py:562-711 */
+int cc_gemm_f64(void* stream, int64_t batch, int64_t M, int64_t N, int64_t K, double alpha,
+                const double* A, int64_t sAb, int64_t sAm, int64_t sAk,
+                const double* B, int64_t sBb, int64_t sBk, int64_t sBn, double beta,
+                double* C, int64_t sCb, int64_t sCm, int64_t sCn);
+
+/* y = alpha * x + beta * y.   NDArrayData.__add__/__iadd__/__mul__  data/__init__.py:93-139 */
+int cc_axpby_f64(void* stream, int64_t n, double alpha, const double* x, double beta, double* y);
+
+/* y[i] = value.   newZeros / newFilled / newTrivial  data/__init__.py:51-90 */
+int cc_fill_f64(void* stream, int64_t n, double value, double* y);
+
+/* y[i*(n+1)] = 1 on an n x n zeroed matrix.   newIdentity  data/__init__.py:59-62 */
+int cc_set_identity_f64(void* stream, int64_t n, double* y);
+
+/* out = x (op) y elementwise, op: 0 mul, 1 div, 2 add, 3 sub.
+ * NDArrayData.__mul__/__truediv__/__add__/__sub__  data/__init__.py:93-139 */
+int cc_binary_f64(void* stream, int op, int64_t n, const double* x, const double* y, double* out);
+
+/* out = f(x) elementwise, op: 0 sqrt, 1 reciprocal where |x| > param (else x), 2 abs,
+ * 3 x * param, 4 reciprocal.   NDArrayData.sqrt and the 1/s cut-off of normalizeAxis
+ * data/__init__.py:287-293, 341-343 */
+int cc_unary_f64(void* stream, int op, int64_t n, const double* x, double param, double* out);
+
+/* out[i][j] = A[i][j] * s[i] (cols = 0) or * s[j] (cols = 1).  The broadcast V*S of
+ * normalizeAxis  data/__init__.py:295-300 */
+int cc_scale_rows_cols_f64(void* stream, int cols, int64_t m, int64_t n, const double* A, const double* s, double* out);
+
+/* out[0] = sum_i x[i] * y[i] (fixed-order).   contractWith over all legs,
+ * computeScalarUsingMultiplier  system/_2d.py:429-431; norm  data/__init__.py:260-262 */
+int cc_dot_f64(void* stream, int64_t n, const double* x, const double* y, double* out);
+
+/* out[0] = number of non-finite entries.   hasNaN  data/__init__.py:209-211 */
+int cc_count_nonfinite_f64(void* stream, int64_t n, const double* x, int64_t* out);
+
+/* dst = (dst_dtype) src.  Precision conversion of slabs and vectors (no reference counterpart). */
+int cc_cast(void* stream, int64_t n, int src_dtype, const void* src, int dst_dtype, void* dst);
+
+/* Thin SVD of a batch of m x n row-major matrices (m >= n, n <= 256) by one-sided
+ * Jacobi: A = U diag(S) Vt with U m x n, S n (descending), Vt n x n.  `work` holds
+ * cc_svd_work_size() doubles.
+ * NDArrayData.svd / normalizeAxis / unitize  data/__init__.py:263-301,344-352; utils.py:879-882 */
+int cc_svd_work_size(int64_t batch, int64_t m, int64_t n, int64_t* out);
+int cc_svd_f64(void* stream, int64_t batch, int64_t m, int64_t n, const double* A,
+               double* U, double* S, double* Vt, double* work);
+
+/* ---- stage 3: the center-site effective-Hamiltonian matvec ----------------------
+ * Y = sum_t (A_{a(t)} (x) B_{b(t)} (x) O_{o(t)}) v, summed over alpha, with
+ *   A_a[alpha][p'][p] (P x P slabs), B_b[alpha][q'][q] (Q x Q slabs), v[p][q][s] (P,Q,d),
+ *   O_o[s'][s] (d x d, o = -1 means identity).
+ * formNormalizationStage3 / formDenseStage3_multiply  tensors/_2d/dense.py:115-173,
+ * summed as multiplyExpectation  tensors/_2d/sparse.py:129-133.
+ * Generic path: any P, Q, d, FP64, via cc_gemm_f64-class kernels; `work` must hold
+ * n_alpha*P*Q*d doubles. */
+int cc_stage3_matvec_generic_work_size(int64_t n_alpha, int64_t P, int64_t Q, int64_t d, int64_t* out);
+int cc_stage3_matvec_generic_f64(void* stream, int n_terms, const double* const* A_slabs,
+                                 const double* const* B_slabs, const double* const* O_mats,
+                                 int64_t n_alpha, int64_t P, int64_t Q, int64_t d,
+                                 const double* v, double* y, double* work);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

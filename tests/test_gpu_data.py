@@ -1,0 +1,86 @@
+"""GPU parity of the device Data protocol (HipData) against NumPy / the oracle."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rng(seed=0):
+    return np.random.default_rng(seed)
+
+
+def test_contract_with_matches_tensordot(gpu):
+    from carcassonne_amd.data import HipData
+    rng = _rng(1)
+    cases = [((3, 4, 5), (5, 4, 2), (1, 2), (1, 0)), ((6, 7), (7, 3), (1,), (0,)), ((2, 3, 4, 5), (4, 2, 6), (0, 2), (1, 0)),
+             ((5, 3), (2, 4), (), ()), ((4, 4, 4), (4, 4, 4), (0, 1, 2), (2, 1, 0)), ((70, 65), (65, 130), (1,), (0,)),
+             ((3, 1, 2, 5), (5, 3), (3, 0), (0, 1))]
+    for sa, sb, aa, ab in cases:
+        a, b = rng.standard_normal(sa), rng.standard_normal(sb)
+        got = HipData.fromArray(a).contractWith(HipData.fromArray(b), aa, ab).toArray()
+        want = np.tensordot(a, b, (aa, ab))
+        assert got.shape == want.shape
+        np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+
+
+def test_join_transpose_and_elementwise(gpu):
+    from carcassonne_amd.data import HipData
+    rng = _rng(2)
+    a = rng.standard_normal((2, 3, 4, 5))
+    h = HipData.fromArray(a)
+    np.testing.assert_array_equal(h.join((0, 2), 3, 1).toArray(), a.transpose(0, 2, 3, 1).reshape(8, 5, 3))
+    np.testing.assert_array_equal(h.transpose(3, 1, 0, 2).toArray(), a.transpose(3, 1, 0, 2))
+    b = rng.standard_normal(a.shape)
+    hb = HipData.fromArray(b)
+    np.testing.assert_allclose((h + hb).toArray(), a + b, rtol=0, atol=0)
+    np.testing.assert_allclose((h * hb).toArray(), a * b, rtol=0, atol=0)
+    np.testing.assert_allclose((h * 2.5).toArray(), a * 2.5, rtol=0, atol=0)
+    h2 = h.copy()
+    h2 += hb
+    np.testing.assert_allclose(h2.toArray(), a + b, rtol=0, atol=0)
+    assert abs(h.norm() - np.linalg.norm(a)) <= 1e-12 * np.linalg.norm(a)
+    assert not h.hasNaN()
+
+
+def test_svd_and_normalize_axis(gpu):
+    from carcassonne_amd.data import HipData
+    from oracle import linalg as OL
+    rng = _rng(3)
+    for m, n in [(20, 6), (432, 6), (64, 64), (7, 1), (300, 37)]:
+        a = rng.standard_normal((m, n))
+        U, S, Vt = HipData.fromArray(a).svd(False)
+        U, S, Vt = U.toArray(), S.toArray(), Vt.toArray()
+        np.testing.assert_allclose(S, np.linalg.svd(a, compute_uv=False), rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose((U * S) @ Vt, a, rtol=1e-11, atol=1e-11)
+        np.testing.assert_allclose(U.T @ U, np.eye(n), atol=1e-12)
+    # normalizeAxis: isometry, normalizer and denormalizer are gauge-invariant products
+    s = rng.standard_normal((3, 2, 3, 2, 2))
+    for axis in range(4):
+        iso, nrm, den = HipData.fromArray(s).normalizeAxis(axis)
+        oiso, onrm, oden = OL.normalize_axis(s, axis)
+        np.testing.assert_allclose(den.toArray(), oden, rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose(nrm.toArray(), onrm, rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose(iso.toArray(), oiso, rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("D,chi,model", [(2, 8, "ising"), (3, 6, "heisenberg"), (4, 8, "ising")])
+def test_stage3_operator_matches_oracle(gpu, D, chi, model):
+    from carcassonne_amd import synthetic as syn
+    from carcassonne_amd.tensors._2d import sparse as PS
+    from carcassonne_amd.data import HipData
+    from oracle import sparse2d
+    from tests.oracle_bridge import to_oracle, to_product, rel_err
+    c, s, st = syn.make_environment(D, chi, model=model, seed=5)
+    op = syn.operator_center(model)
+    oc, os_, oo = to_oracle(c, s, op)
+    ref = sparse2d.form_multipliers(oc, os_, oo)
+    pc, ps, po = to_product(c, s, op)
+    H, N = PS.formExpectationAndNormalizationMultipliers(pc, ps, po)
+    v = HipData.fromArray(st)
+    assert rel_err(H(v).toArray(), ref.apply_H(st)) < 1e-12
+    assert rel_err(N(v).toArray(), ref.apply_N(st)) < 1e-12
+    assert H.cost_of_multiply == ref.cost_H and N.cost_of_multiply == ref.cost_N
+    assert H.cost_of_formMatrix == ref.cost_H_form
+    if D <= 3:
+        assert rel_err(H.formMatrix().toArray(), ref.matrix_H()) < 1e-12
+        assert rel_err(N.formMatrix().toArray(), ref.matrix_N()) < 1e-12
