@@ -16,6 +16,7 @@ CC_F64, CC_F32, CC_F16, CC_BF16 = 0, 1, 2, 3
 
 _vp, _i64, _i32, _f64 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
 _pi64, _pvp = ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_void_p)
+_pi32, _pf64 = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
 
 # name -> argtypes (all return int status)
 SIGNATURES = {
@@ -36,6 +37,22 @@ SIGNATURES = {
     "cc_svd_f64": [_vp, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
     "cc_stage3_matvec_generic_work_size": [_i64, _i64, _i64, _i64, _pi64],
     "cc_stage3_matvec_generic_f64": [_vp, _i32, _pvp, _pvp, _pvp, _i64, _i64, _i64, _i64, _vp, _vp, _vp],
+    "cc_stage3_fused_supported": [_i64, _i64],
+    "cc_stage3_packed_size": [_i64, _i64, _pi64],
+    "cc_stage3_fused_work_size": [_i64, _i64, _i64, _pi64],
+    "cc_stage3_pack_f64": [_vp, _i64, _i64, _vp, _vp],
+    "cc_stage3_plan_create": [ctypes.POINTER(ctypes.c_void_p), _i32, _pvp, _i32, _pvp, _i32, _pi32, _pi32, _pi32,
+                              _i32, _pf64, _i64, _i64, _i64, _vp],
+    "cc_stage3_plan_apply": [_vp, _vp, _vp, _vp],
+    "cc_stage3_plan_destroy": [_vp],
+    "cc_cholesky_f64": [_vp, _i64, _vp, _vp],
+    "cc_trtri_lower_f64": [_vp, _i64, _vp, _vp],
+    "cc_transpose_f64": [_vp, _i64, _vp, _vp],
+    "cc_skinny_f64": [_vp, _i64, _i32, _vp, _i32, _vp, _vp],
+    "cc_lanczos_work_size": [_i64, _i32, _pi64],
+    "cc_lanczos_f64": [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _f64, _f64, _i32, _i32, _vp, _vp, _pf64],
+    "cc_stage3_matvec_fused_f64": [_vp, _i32, _pvp, _i32, _pvp, _i32, _pi32, _pi32, _pi32, _i32, _pf64, _i64, _i64,
+                                   _i64, _vp, _vp, _vp],
 }
 
 _lib = None
@@ -92,6 +109,23 @@ def call(name, *args):
 def i64_array(values):
     values = [int(v) for v in values]
     return (ctypes.c_int64 * max(1, len(values)))(*values)
+
+
+def i32_array(values):
+    values = [int(v) for v in values]
+    return (ctypes.c_int * max(1, len(values)))(*values)
+
+
+def f64_array(values):
+    values = [float(v) for v in values]
+    return (ctypes.c_double * max(1, len(values)))(*values)
+
+
+def query(name, *args):
+    """Call a size / capability query whose last argument is an int64 out-parameter."""
+    out = i64_array([0])
+    call(name, *args, out)
+    return int(out[0])
 
 
 def ptr_array(ptrs):

@@ -38,3 +38,7 @@ static inline unsigned cc_blocks(int64_t n, int threads, unsigned cap = 65535u *
   if (b > (int64_t)cap) b = cap;
   return (unsigned)b;
 }
+
+// Internal (not exported through the C ABI header): operator plans used by the solvers.
+int cc_internal_stage3_apply(hipStream_t st, const void* plan, const double* x, double* y);
+int64_t cc_internal_stage3_n(const void* plan);

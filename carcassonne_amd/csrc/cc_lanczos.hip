@@ -1,0 +1,263 @@
+// Lanczos for the lowest eigenpair of the pencil (H, N), N = (L L^T) (x) I_d, on the whitened
+// operator W H W^T (W = L^-1 (x) I_d), with full (twice-applied) reorthogonalisation.  All
+// vectors and recurrence scalars stay in device memory; the host loop only reads two doubles
+// (Ritz value and residual estimate) every `check_every` steps.
+#include "cc_common.h"
+
+#include <cmath>
+#include <vector>
+
+namespace {
+
+__global__ void __launch_bounds__(256) lz_coeff_kernel(int64_t n, const double* __restrict__ Q, const double* __restrict__ w,
+                                                       double* __restrict__ c) {
+  __shared__ double red[256];
+  const double* q = Q + (int64_t)blockIdx.x * n;
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 256) acc += q[i] * w[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) c[blockIdx.x] = red[0];
+}
+
+// w[i] -= sum_j c[j] Q[j][i]   (or z[i] = sum_j c[j] Q[j][i] when `assign`)
+__global__ void lz_combine_kernel(int m, int64_t n, const double* __restrict__ Q, const double* __restrict__ c,
+                                  double* __restrict__ w, int assign) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int j = 0; j < m; ++j) acc += c[j] * Q[(int64_t)j * n + i];
+    w[i] = assign ? acc : w[i] - acc;
+  }
+}
+
+// w -= alpha q_j + beta q_{j-1}  (alpha, beta read from device memory)
+__global__ void lz_three_term_kernel(int64_t n, double* __restrict__ w, const double* __restrict__ qj,
+                                     const double* __restrict__ qjm1, const double* __restrict__ alpha,
+                                     const double* __restrict__ beta) {
+  const double a = *alpha, b = beta ? *beta : 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    w[i] -= a * qj[i] + (beta ? b * qjm1[i] : 0.0);
+}
+
+// beta = ||w||, q = w / beta (single 1024-thread block, fixed-order tree)
+__global__ void __launch_bounds__(1024) lz_normalize_kernel(int64_t n, const double* __restrict__ w, double* __restrict__ q,
+                                                            double* __restrict__ beta) {
+  __shared__ double red[1024];
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) acc += w[i] * w[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  const double b = sqrt(red[0]);
+  const double inv = b > 0.0 ? 1.0 / b : 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) q[i] = w[i] * inv;
+  if (threadIdx.x == 0 && beta) *beta = b;
+}
+
+__device__ int sturm_count(int m, const double* a, const double* b, double x, double pivmin) {
+  int cnt = 0;
+  double q = a[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  cnt += q < 0.0;
+  for (int i = 1; i < m; ++i) {
+    q = a[i] - x - b[i - 1] * b[i - 1] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    cnt += q < 0.0;
+  }
+  return cnt;
+}
+
+// Lowest eigenpair of the m x m tridiagonal (a, b): multisection over one wave, then inverse
+// iteration (LU with partial pivoting) on lane 0.  out[0] = theta, out[1] = |b[m-1] * s[m-1]|.
+__global__ void __launch_bounds__(64) lz_tridiag_kernel(int m, const double* __restrict__ a, const double* __restrict__ b,
+                                                        double* __restrict__ out, double* __restrict__ s,
+                                                        double* __restrict__ scratch) {
+  const int lane = threadIdx.x;
+  double lo = INFINITY, hi = -INFINITY, tnorm = 0.0;
+  for (int i = lane; i < m; i += 64) {
+    double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < m - 1 ? fabs(b[i]) : 0.0);
+    lo = fmin(lo, a[i] - r);
+    hi = fmax(hi, a[i] + r);
+    tnorm = fmax(tnorm, fabs(a[i]) + r);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, o, 64));
+    hi = fmax(hi, __shfl_xor(hi, o, 64));
+    tnorm = fmax(tnorm, __shfl_xor(tnorm, o, 64));
+  }
+  const double eps = 2.220446049250313e-16;
+  const double pivmin = fmax(tnorm * 1e-300, 1e-300) + 4.9e-324;
+  lo -= 2.0 * eps * tnorm + pivmin;
+  hi += 2.0 * eps * tnorm + pivmin;
+  for (int round = 0; round < 20; ++round) {
+    double x = lo + (hi - lo) * (double)(lane + 1) / 65.0;
+    int cnt = sturm_count(m, a, b, x, pivmin);
+    unsigned long long mask = __ballot(cnt >= 1);
+    if (mask == 0ULL) {
+      lo = lo + (hi - lo) * 64.0 / 65.0;
+    } else {
+      int first = __ffsll((long long)mask) - 1;
+      double nhi = lo + (hi - lo) * (double)(first + 1) / 65.0;
+      double nlo = first == 0 ? lo : lo + (hi - lo) * (double)first / 65.0;
+      lo = nlo;
+      hi = nhi;
+    }
+    if (hi - lo <= 2.0 * eps * fmax(fabs(lo), fabs(hi)) + pivmin) break;
+  }
+  const double theta = 0.5 * (lo + hi);
+  if (lane != 0) return;
+  double* d = scratch;
+  double* dl = scratch + m;
+  double* du = scratch + 2 * m;
+  double* du2 = scratch + 3 * m;
+  double* piv = scratch + 4 * m;
+  const double tiny = eps * fmax(tnorm, 1e-300);
+  for (int i = 0; i < m; ++i) s[i] = 1.0 + 0.01 * (double)((i * 7919) % 101) / 101.0;
+  for (int it = 0; it < 3; ++it) {
+    for (int i = 0; i < m; ++i) {
+      d[i] = a[i] - theta;
+      if (i < m - 1) { dl[i] = b[i]; du[i] = b[i]; }
+      du2[i] = 0.0;
+      piv[i] = 0.0;
+    }
+    for (int i = 0; i < m - 1; ++i) {
+      if (fabs(d[i]) >= fabs(dl[i])) {
+        if (d[i] == 0.0) d[i] = tiny;
+        double f = dl[i] / d[i];
+        dl[i] = f;
+        d[i + 1] -= f * du[i];
+      } else {
+        double f = d[i] / dl[i];
+        d[i] = dl[i];
+        dl[i] = f;
+        double t = du[i];
+        du[i] = d[i + 1];
+        d[i + 1] = t - f * d[i + 1];
+        if (i < m - 2) {
+          du2[i] = du[i + 1];
+          du[i + 1] = -f * du[i + 1];
+        }
+        piv[i] = 1.0;
+      }
+    }
+    if (d[m - 1] == 0.0) d[m - 1] = tiny;
+    for (int i = 0; i < m - 1; ++i) {
+      if (piv[i] == 0.0) {
+        s[i + 1] -= dl[i] * s[i];
+      } else {
+        double t = s[i];
+        s[i] = s[i + 1];
+        s[i + 1] = t - dl[i] * s[i];
+      }
+    }
+    s[m - 1] /= d[m - 1];
+    if (m > 1) s[m - 2] = (s[m - 2] - du[m - 2] * s[m - 1]) / d[m - 2];
+    for (int i = m - 3; i >= 0; --i) s[i] = (s[i] - du[i] * s[i + 1] - du2[i] * s[i + 2]) / d[i];
+    double nrm = 0.0, big = 0.0;
+    for (int i = 0; i < m; ++i) big = fmax(big, fabs(s[i]));
+    for (int i = 0; i < m; ++i) { s[i] /= big; nrm += s[i] * s[i]; }
+    nrm = sqrt(nrm);
+    for (int i = 0; i < m; ++i) s[i] /= nrm;
+  }
+  out[0] = theta;
+  out[1] = fabs(b[m - 1] * s[m - 1]);
+}
+
+struct Whitening {
+  int64_t n_env, d;
+  const double *L, *Lt, *W, *Wt;
+};
+
+int op_whitened(hipStream_t st, const void* plan, const Whitening& wh, const double* q, double* t1, double* t2, double* w) {
+  int rc = cc_skinny_f64(st, wh.n_env, 2, wh.Wt, (int)wh.d, q, t1);  // t1 = W^T q
+  if (rc) return rc;
+  rc = cc_internal_stage3_apply(st, plan, t1, t2);                   // t2 = H t1
+  if (rc) return rc;
+  return cc_skinny_f64(st, wh.n_env, 1, wh.W, (int)wh.d, t2, w);     // w = W t2
+}
+}  // namespace
+
+extern "C" int cc_lanczos_work_size(int64_t n, int max_iter, int64_t* out) {
+  *out = (int64_t)(max_iter + 1) * n + 3 * n + 12 * (int64_t)(max_iter + 2) + 16;
+  return CC_OK;
+}
+
+extern "C" int cc_lanczos_f64(void* stream, const void* plan, int64_t n_env, int64_t d, const double* L, const double* Lt,
+                              const double* W, const double* Wt, const double* x0, double tol_residual,
+                              double tol_energy, int max_iter, int check_every, double* work, double* x_out,
+                              double* host_result /* theta, residual, iterations */) {
+  hipStream_t st = cc_stream(stream);
+  const int64_t n = n_env * d;
+  if (!plan || n <= 0 || max_iter < 1 || check_every < 1) { cc_set_last_error("lanczos: bad arguments"); return CC_ERR_ARG; }
+  if (cc_internal_stage3_n(plan) != n) { cc_set_last_error("lanczos: operator size does not match the whitening"); return CC_ERR_DIM; }
+  Whitening wh{n_env, d, L, Lt, W, Wt};
+  double* Q = work;
+  double* w = Q + (int64_t)(max_iter + 1) * n;
+  double* t1 = w + n;
+  double* t2 = t1 + n;
+  double* alpha = t2 + n;
+  double* beta = alpha + (max_iter + 2);
+  double* c = beta + (max_iter + 2);
+  double* s = c + (max_iter + 2);
+  double* scratch = s + (max_iter + 2);  // 5 * (max_iter + 2)
+  double* res = scratch + 5 * (max_iter + 2);
+  const unsigned gb = cc_blocks(n, 256);
+  int rc;
+
+  // q0 = L^T x0 / ||L^T x0||
+  rc = cc_skinny_f64(st, n_env, 2, Lt, (int)d, x0, w);
+  if (rc) return rc;
+  lz_normalize_kernel<<<1, 1024, 0, st>>>(n, w, Q, nullptr);
+  CC_LAUNCH_CHECK();
+
+  double prev_theta = NAN, theta = NAN, resid = NAN;
+  int m = 0;
+  for (int j = 0; j < max_iter; ++j) {
+    double* qj = Q + (int64_t)j * n;
+    rc = op_whitened(st, plan, wh, qj, t1, t2, w);
+    if (rc) return rc;
+    rc = cc_dot_f64(st, n, qj, w, alpha + j);
+    if (rc) return rc;
+    lz_three_term_kernel<<<gb, 256, 0, st>>>(n, w, qj, j > 0 ? qj - n : nullptr, alpha + j, j > 0 ? beta + j - 1 : nullptr);
+    CC_LAUNCH_CHECK();
+    for (int pass = 0; pass < 2; ++pass) {
+      lz_coeff_kernel<<<j + 1, 256, 0, st>>>(n, Q, w, c);
+      CC_LAUNCH_CHECK();
+      lz_combine_kernel<<<gb, 256, 0, st>>>(j + 1, n, Q, c, w, 0);
+      CC_LAUNCH_CHECK();
+    }
+    lz_normalize_kernel<<<1, 1024, 0, st>>>(n, w, Q + (int64_t)(j + 1) * n, beta + j);
+    CC_LAUNCH_CHECK();
+    m = j + 1;
+    if (m % check_every == 0 || m == max_iter || m == n) {
+      lz_tridiag_kernel<<<1, 64, 0, st>>>(m, alpha, beta, res, s, scratch);
+      CC_LAUNCH_CHECK();
+      double h[2];
+      CC_HIP(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, st));
+      CC_HIP(hipStreamSynchronize(st));
+      theta = h[0];
+      resid = h[1];
+      bool done = resid <= tol_residual * fabs(theta) || m == n ||
+                  (!std::isnan(prev_theta) && prev_theta - theta <= tol_energy * fabs(theta));
+      prev_theta = theta;
+      if (done) break;
+    }
+  }
+  // Ritz vector z = Q s, x = W^T z   (N-normalised because ||z|| = 1)
+  lz_combine_kernel<<<gb, 256, 0, st>>>(m, n, Q, s, t2, 1);
+  CC_LAUNCH_CHECK();
+  rc = cc_skinny_f64(st, n_env, 2, Wt, (int)d, t2, x_out);
+  if (rc) return rc;
+  CC_HIP(hipStreamSynchronize(st));
+  host_result[0] = theta;
+  host_result[1] = resid;
+  host_result[2] = (double)m;
+  return CC_OK;
+}

@@ -1,0 +1,317 @@
+// Fused stage-3 matvec on CDNA4 matrix cores (FP64, v_mfma_f64_16x16x4f64).
+//
+//   Y[p'][q'][s'] = sum_t sum_alpha sum_{p,q,s} A_a(t)[alpha][p'][p] B_b(t)[alpha][q'][q] O_o(t)[s'][s] v[p][q][s]
+//
+// One wave owns a contiguous range of alpha.  Per alpha and per distinct A channel a:
+//   stage A   W_a[idx = s*P4 + q][p'] = sum_p V^T[idx][p] A_a[p'][p]      (V^T operands stay in registers)
+// and per term t of that channel:
+//   stage B   Y[p'][q'][s'] += sum_q (sum_s O[s'][s] W_a[s*P4+q][p']) B_b[q'][q]
+// The f64 MFMA result layout (D[i][j] in lane j + 16(i%4), register i/4) puts W_a[s*P4 + 4u + lane/16][p']
+// in the lane/register that stage B needs as its A operand X[p' = lane%16][q = 4u + lane/16], so the
+// intermediate never leaves registers.  Slabs are pre-packed in MFMA-operand order
+// (packed[alpha][u][n][lane] = S[16n + lane%16][4u + lane/16], zero-padded), so each operand load is one
+// coalesced 512-byte read.  Each 4-wave workgroup reduces its waves' Y through LDS in a fixed order and
+// writes one partial; a second kernel sums the partials in a fixed order.
+#include "cc_common.h"
+
+namespace {
+constexpr int MAX_CH = 16, MAX_TERMS = 48, MAX_OPS = 16, MAX_D = 4;
+
+struct FusedPlan {
+  const double* A[MAX_CH];
+  const double* B[MAX_CH];
+  int n_terms;
+  int ta[MAX_TERMS], tb[MAX_TERMS], to[MAX_TERMS];  // terms sorted by channel a; to = -1 -> identity
+  double ops[MAX_OPS][MAX_D * MAX_D];
+  int64_t n_alpha;
+};
+
+template <int P, int DP>
+struct Shape {
+  static constexpr int P4 = (P + 3) / 4 * 4;      // padded p / q extent (multiple of 4)
+  static constexpr int NT = (P + 15) / 16;         // 16-wide tiles over p' and q'
+  static constexpr int KS = P4 / 4;                // 4-deep k steps over p (stage A) and q (stage B)
+  static constexpr int IDX = DP * P4;              // stage-A rows idx = s*P4 + q
+  static constexpr int MT = (IDX + 15) / 16;       // stage-A row tiles
+  static constexpr int SLAB = KS * NT * 64;        // packed doubles per alpha
+};
+
+__global__ void pack_kernel(int64_t n_alpha, int P, int NT, int KS, const double* __restrict__ src, double* __restrict__ dst) {
+  const int64_t per = (int64_t)KS * NT * 64;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n_alpha * per; e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t a = e / per;
+    int r = (int)(e % per);
+    int lane = r & 63, t = r >> 6;
+    int n = t % NT, u = t / NT;
+    int row = 16 * n + (lane & 15), col = 4 * u + (lane >> 4);
+    dst[e] = (row < P && col < P) ? src[(a * P + row) * P + col] : 0.0;
+  }
+}
+
+template <int P, int DP>
+__global__ void __launch_bounds__(256) fused_f64_kernel(const FusedPlan plan, const double* __restrict__ v,
+                                                        double* __restrict__ partial) {
+  using S = Shape<P, DP>;
+  constexpr int P4 = S::P4, NT = S::NT, KS = S::KS, MT = S::MT, SLAB = S::SLAB;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t gw = (int64_t)blockIdx.x * 4 + wave, nw = (int64_t)gridDim.x * 4;
+  const int64_t a0 = plan.n_alpha * gw / nw, a1 = plan.n_alpha * (gw + 1) / nw;
+
+  // stage-A A operands V^T[idx][p] (idx = 16m + lane%16, p = 4u + lane/16), stored in LDS in lane order so
+  // each read is conflict-free; the same LDS is reused for the final reduction.
+  constexpr int NY = P * P * DP;
+  constexpr int NVT = MT * KS * 64;
+  constexpr int NLDS = (NVT > 4 * NY) ? NVT : 4 * NY;
+  __shared__ double lds[NLDS];
+  for (int e = threadIdx.x; e < NVT; e += 256) {
+    int l = e & 63, mu = e >> 6, u = mu % KS, m = mu / KS;
+    int idx = 16 * m + (l & 15), p = 4 * u + (l >> 4);
+    int s = idx / P4, q = idx % P4;
+    lds[e] = (s < DP && q < P && p < P) ? v[(p * P + q) * DP + s] : 0.0;
+  }
+  __syncthreads();
+
+  f64x4 y[NT][NT][DP];
+#pragma unroll
+  for (int i = 0; i < NT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+#pragma unroll
+      for (int s = 0; s < DP; ++s) y[i][j][s] = (f64x4){0.0, 0.0, 0.0, 0.0};
+
+  for (int64_t al = a0; al < a1; ++al) {
+    int t = 0;
+    while (t < plan.n_terms) {
+      const int a = plan.ta[t];
+      const double* Ap = plan.A[a] + al * SLAB;
+      const int t_begin = t;
+      int t_end = t;
+      while (t_end < plan.n_terms && plan.ta[t_end] == a) ++t_end;
+#pragma unroll
+      for (int np = 0; np < NT; ++np) {
+        // stage A for the p' tile np: w[m] = V^T[16m.., :] A_a^T[:, 16np..]
+        f64x4 w[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) w[m] = (f64x4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int u = 0; u < KS; ++u) {
+          double bop = Ap[(u * NT + np) * 64 + lane];
+#pragma unroll
+          for (int m = 0; m < MT; ++m)
+            w[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(lds[(m * KS + u) * 64 + lane], bop, w[m], 0, 0, 0);
+        }
+        // stage B for every term of channel a, p' tile np
+        for (int tt = t_begin; tt < t_end; ++tt) {
+          const double* Bp = plan.B[plan.tb[tt]] + al * SLAB;
+          const int o = plan.to[tt];
+          double oc[DP][DP];
+#pragma unroll
+          for (int i = 0; i < DP; ++i)
+#pragma unroll
+            for (int j = 0; j < DP; ++j) oc[i][j] = (o < 0) ? (i == j ? 1.0 : 0.0) : plan.ops[o < 0 ? 0 : o][i * DP + j];
+#pragma unroll
+          for (int u = 0; u < KS; ++u) {
+            double xs[DP];
+#pragma unroll
+            for (int s = 0; s < DP; ++s) {
+              const int idx = s * P4 + 4 * u;  // the lane's row within the group is idx + lane/16
+              xs[s] = w[idx / 16][(idx % 16) / 4];
+            }
+            double x[DP];
+#pragma unroll
+            for (int sp = 0; sp < DP; ++sp) {
+              double acc = 0.0;
+#pragma unroll
+              for (int s = 0; s < DP; ++s) acc += oc[sp][s] * xs[s];
+              x[sp] = acc;
+            }
+#pragma unroll
+            for (int nq = 0; nq < NT; ++nq) {
+              double bop = Bp[(u * NT + nq) * 64 + lane];
+#pragma unroll
+              for (int sp = 0; sp < DP; ++sp)
+                y[np][nq][sp] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[sp], bop, y[np][nq][sp], 0, 0, 0);
+            }
+          }
+        }
+      }
+      t = t_end;
+    }
+  }
+
+  // workgroup reduction in a fixed order: wave w writes slot w, then every thread sums slots 0..3
+  __syncthreads();
+  double* red0 = lds;
+#pragma unroll
+  for (int np = 0; np < NT; ++np)
+#pragma unroll
+    for (int nq = 0; nq < NT; ++nq)
+#pragma unroll
+      for (int s = 0; s < DP; ++s)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int pp = 16 * np + (lane >> 4) + 4 * r, qq = 16 * nq + (lane & 15);
+          if (pp < P && qq < P) red0[wave * NY + (pp * P + qq) * DP + s] = y[np][nq][s][r];
+        }
+  __syncthreads();
+  for (int i = threadIdx.x; i < NY; i += 256)
+    partial[(int64_t)blockIdx.x * NY + i] = ((red0[i] + red0[NY + i]) + red0[2 * NY + i]) + red0[3 * NY + i];
+}
+
+__global__ void reduce_partials_kernel(int64_t groups, int64_t n, const double* __restrict__ partial, double* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int64_t g = 0; g < groups; ++g) acc += partial[g * n + i];
+    y[i] = acc;
+  }
+}
+
+template <int P, int DP>
+int launch_fused(hipStream_t st, const FusedPlan& plan, const double* v, double* y, double* work, int groups) {
+  fused_f64_kernel<P, DP><<<groups, 256, 0, st>>>(plan, v, work);
+  CC_LAUNCH_CHECK();
+  const int64_t n = (int64_t)P * P * DP;
+  reduce_partials_kernel<<<cc_blocks(n, 256), 256, 0, st>>>(groups, n, work, y);
+  CC_LAUNCH_CHECK();
+  return CC_OK;
+}
+
+int fused_groups(int64_t n_alpha) {
+  // four alpha per wave at the largest shapes; never more workgroups than waves of work
+  int64_t g = (n_alpha + 15) / 16;
+  if (g > 1024) g = 1024;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+}  // namespace
+
+namespace {
+// A persistent operator: the plan is validated and copied once, then applied many times.
+struct Stage3PlanHandle {
+  FusedPlan plan;
+  int64_t P, d;
+  int groups;
+  double* work;
+};
+
+int apply_plan(hipStream_t st, const Stage3PlanHandle* h, const double* v, double* y) {
+  switch (h->P) {
+    case 4: return launch_fused<4, 2>(st, h->plan, v, y, h->work, h->groups);
+    case 9: return launch_fused<9, 2>(st, h->plan, v, y, h->work, h->groups);
+    case 16: return launch_fused<16, 2>(st, h->plan, v, y, h->work, h->groups);
+    case 25: return launch_fused<25, 2>(st, h->plan, v, y, h->work, h->groups);
+    case 36: return launch_fused<36, 2>(st, h->plan, v, y, h->work, h->groups);
+  }
+  cc_set_last_error("fused stage-3: shape not instantiated");
+  return CC_ERR_UNSUPPORTED;
+}
+
+int fill_plan(FusedPlan& plan, int n_a, const double* const* A_packed, int n_b, const double* const* B_packed,
+              int n_terms, const int* term_a, const int* term_b, const int* term_o, int n_ops, const double* host_ops,
+              int64_t n_alpha, int64_t d) {
+  if (n_a < 1 || n_a > MAX_CH || n_b < 1 || n_b > MAX_CH || n_terms < 1 || n_terms > MAX_TERMS || n_ops < 0 ||
+      n_ops > MAX_OPS || n_alpha < 1 || d < 1 || d > MAX_D) {
+    cc_set_last_error("fused stage-3: plan exceeds kernel limits");
+    return CC_ERR_UNSUPPORTED;
+  }
+  plan = FusedPlan{};
+  for (int i = 0; i < n_a; ++i) plan.A[i] = A_packed[i];
+  for (int i = 0; i < n_b; ++i) plan.B[i] = B_packed[i];
+  plan.n_terms = n_terms;
+  for (int t = 0; t < n_terms; ++t) {
+    if (term_a[t] < 0 || term_a[t] >= n_a || term_b[t] < 0 || term_b[t] >= n_b || term_o[t] < -1 || term_o[t] >= n_ops) {
+      cc_set_last_error("fused stage-3: term index out of range");
+      return CC_ERR_ARG;
+    }
+    if (t > 0 && term_a[t] < term_a[t - 1]) { cc_set_last_error("fused stage-3: terms must be sorted by A channel"); return CC_ERR_ARG; }
+    plan.ta[t] = term_a[t];
+    plan.tb[t] = term_b[t];
+    plan.to[t] = term_o[t];
+  }
+  for (int o = 0; o < n_ops; ++o)
+    for (int e = 0; e < d * d; ++e) plan.ops[o][e] = host_ops[o * d * d + e];
+  plan.n_alpha = n_alpha;
+  return CC_OK;
+}
+}  // namespace
+
+int cc_internal_stage3_apply(hipStream_t st, const void* plan, const double* x, double* y) {
+  return apply_plan(st, static_cast<const Stage3PlanHandle*>(plan), x, y);
+}
+int64_t cc_internal_stage3_n(const void* plan) {
+  auto h = static_cast<const Stage3PlanHandle*>(plan);
+  return h->P * h->P * h->d;
+}
+
+extern "C" int cc_stage3_fused_supported(int64_t P, int64_t d) {
+  return (d == 2 && (P == 4 || P == 9 || P == 16 || P == 25 || P == 36)) ? 1 : 0;
+}
+
+extern "C" int cc_stage3_packed_size(int64_t n_alpha, int64_t P, int64_t* out) {
+  int64_t NT = (P + 15) / 16, KS = (P + 3) / 4;
+  *out = n_alpha * KS * NT * 64;
+  return CC_OK;
+}
+
+extern "C" int cc_stage3_fused_work_size(int64_t n_alpha, int64_t P, int64_t d, int64_t* out) {
+  *out = (int64_t)fused_groups(n_alpha) * P * P * d;
+  return CC_OK;
+}
+
+extern "C" int cc_stage3_pack_f64(void* stream, int64_t n_alpha, int64_t P, const double* slabs, double* packed) {
+  if (n_alpha < 0 || P <= 0) { cc_set_last_error("pack: bad sizes"); return CC_ERR_ARG; }
+  int NT = (int)((P + 15) / 16), KS = (int)((P + 3) / 4);
+  int64_t total = n_alpha * KS * NT * 64;
+  if (total == 0) return CC_OK;
+  pack_kernel<<<cc_blocks(total, 256), 256, 0, cc_stream(stream)>>>(n_alpha, (int)P, NT, KS, slabs, packed);
+  CC_LAUNCH_CHECK();
+  return CC_OK;
+}
+
+extern "C" int cc_stage3_matvec_fused_f64(void* stream, int n_a, const double* const* A_packed, int n_b,
+                                          const double* const* B_packed, int n_terms, const int* term_a,
+                                          const int* term_b, const int* term_o, int n_ops, const double* host_ops,
+                                          int64_t n_alpha, int64_t P, int64_t d, const double* v, double* y,
+                                          double* work) {
+  if (!cc_stage3_fused_supported(P, d)) { cc_set_last_error("fused stage-3: shape not instantiated"); return CC_ERR_UNSUPPORTED; }
+  FusedPlan plan;
+  int rc = fill_plan(plan, n_a, A_packed, n_b, B_packed, n_terms, term_a, term_b, term_o, n_ops, host_ops, n_alpha, d);
+  if (rc) return rc;
+  hipStream_t st = cc_stream(stream);
+  int groups = fused_groups(n_alpha);
+  switch (P) {
+    case 4: return launch_fused<4, 2>(st, plan, v, y, work, groups);
+    case 9: return launch_fused<9, 2>(st, plan, v, y, work, groups);
+    case 16: return launch_fused<16, 2>(st, plan, v, y, work, groups);
+    case 25: return launch_fused<25, 2>(st, plan, v, y, work, groups);
+    case 36: return launch_fused<36, 2>(st, plan, v, y, work, groups);
+  }
+  return CC_ERR_UNSUPPORTED;
+}
+
+extern "C" int cc_stage3_plan_create(void** out, int n_a, const double* const* A_packed, int n_b,
+                                     const double* const* B_packed, int n_terms, const int* term_a, const int* term_b,
+                                     const int* term_o, int n_ops, const double* host_ops, int64_t n_alpha, int64_t P,
+                                     int64_t d, double* work) {
+  *out = nullptr;
+  if (!cc_stage3_fused_supported(P, d)) { cc_set_last_error("fused stage-3: shape not instantiated"); return CC_ERR_UNSUPPORTED; }
+  Stage3PlanHandle* h = new Stage3PlanHandle();
+  int rc = fill_plan(h->plan, n_a, A_packed, n_b, B_packed, n_terms, term_a, term_b, term_o, n_ops, host_ops, n_alpha, d);
+  if (rc) { delete h; return rc; }
+  h->P = P;
+  h->d = d;
+  h->groups = fused_groups(n_alpha);
+  h->work = work;
+  *out = h;
+  return CC_OK;
+}
+
+extern "C" int cc_stage3_plan_apply(void* stream, const void* plan, const double* x, double* y) {
+  if (!plan) { cc_set_last_error("null plan"); return CC_ERR_ARG; }
+  return apply_plan(cc_stream(stream), static_cast<const Stage3PlanHandle*>(plan), x, y);
+}
+
+extern "C" int cc_stage3_plan_destroy(void* plan) {
+  delete static_cast<Stage3PlanHandle*>(plan);
+  return CC_OK;
+}

@@ -1,0 +1,211 @@
+"""Device eigensolvers of the center-site solve.
+
+* GroundStateSolver -- the residual-controlled FP64 solve on the GPU: the
+  environment norm matrix N_env (N = N_env (x) I_d) is formed with one MFMA GEMM
+  over the identity-channel slabs, factored N_env = L L^T (blocked Cholesky),
+  inverted (W = L^-1), and the lowest eigenpair of W H W^T is found by Lanczos
+  with full reorthogonalisation (cc_lanczos_f64: native driver, all vectors and
+  recurrence scalars on the device).
+* relax_over -- relaxOver (carcassonne/utils.py:805-878) on device vectors: the same
+  restarted Arnoldi with Krylov size 3, classical Gram-Schmidt, 1e-14 completeness
+  cut-off, projected eig with argmin Re, stop rule, dense-vs-matrix-free choice by
+  the Multiplier cost rule, final Rayleigh quotient and RelaxFailed test.  N^-1 is
+  applied through a Cholesky factor of the formed normalization matrix (the
+  reference uses an LU, or GMRES when forming is dearer; both solve the same
+  system).
+"""
+import ctypes
+
+import numpy as np
+import torch
+from scipy.linalg import eig
+
+from . import _lib
+from .data import HipData
+from .device import empty, ptr, stream, synchronize
+from .errors import RelaxFailed
+
+
+class NormalizationFactor:
+    """N_env = L L^T and W = L^-1 for a normalization Stage3Operator (one identity term)."""
+
+    def __init__(self, n_operator):
+        if len(n_operator._A) != 1 or n_operator._O[0] is not None:
+            raise ValueError("the normalization operator must be a single identity term")
+        A, B = n_operator._A[0], n_operator._B[0]
+        P, Q, na = n_operator.P, n_operator.Q, n_operator.n_alpha
+        self.n_env, self.d = P * Q, n_operator.d
+        G = empty((P * P, Q * Q))
+        # G[(p'p)][(q'q)] = sum_alpha A[alpha][(p'p)] B[alpha][(q'q)]
+        _lib.call("cc_gemm_f64", stream(), 1, P * P, Q * Q, na, 1.0, ptr(A._t), 0, 1, P * P, ptr(B._t), 0, Q * Q, 1,
+                  0.0, ptr(G), 0, Q * Q, 1)
+        Nenv = empty((self.n_env, self.n_env))
+        _lib.call("cc_permute", stream(), _lib.CC_F64, ptr(G), ptr(Nenv), 4, _lib.i64_array([P, Q, P, Q]),
+                  _lib.i64_array([P * Q * Q, Q, Q * Q, 1]))
+        self.N_env = Nenv
+        L = Nenv.clone()
+        info = torch.zeros(1, dtype=torch.int32, device=L.device)
+        _lib.call("cc_cholesky_f64", stream(), self.n_env, ptr(L), ptr(info))
+        if int(info.item()) != 0:
+            raise ValueError("the normalization matrix is not positive definite (pivot {})".format(int(info.item())))
+        self.L = L
+        self.Lt = empty(L.shape)
+        _lib.call("cc_transpose_f64", stream(), self.n_env, ptr(L), ptr(self.Lt))
+        self.W = empty(L.shape)
+        _lib.call("cc_trtri_lower_f64", stream(), self.n_env, ptr(L), ptr(self.W))
+        self.Wt = empty(L.shape)
+        _lib.call("cc_transpose_f64", stream(), self.n_env, ptr(self.W), ptr(self.Wt))
+
+
+class GroundStateSolver:
+    """Lowest eigenpair of (H, N) for one environment, FP64 on the GPU."""
+
+    def __init__(self, h_operator, n_operator, factor=None):
+        if not h_operator.use_fused:
+            raise NotImplementedError("the device Lanczos needs an operator with a fused kernel "
+                                      "(P = Q in {4, 9, 16, 25, 36}, d = 2)")
+        self.H, self.N = h_operator, n_operator
+        self.factor = factor if factor is not None else NormalizationFactor(n_operator)
+        self.n = h_operator.n
+        h_operator._ensure_fused()
+        self._plan = h_operator.plan_handle()
+        self._work = None
+        self._max_iter = 0
+
+    def solve(self, x0, tol_residual=1e-10, tol_energy=1e-13, max_iter=400, check_every=4):
+        """x0: flat device tensor (or HipData); returns (x, theta, residual, iterations)."""
+        if isinstance(x0, HipData):
+            x0 = x0._t.reshape(-1)
+        max_iter = int(min(max_iter, self.n))
+        if self._work is None or self._max_iter < max_iter:
+            self._work = empty((_lib.query("cc_lanczos_work_size", self.n, max_iter),))
+            self._max_iter = max_iter
+        x = empty((self.n,))
+        res = _lib.f64_array([0, 0, 0])
+        f = self.factor
+        _lib.call("cc_lanczos_f64", stream(), self._plan, f.n_env, f.d, ptr(f.L), ptr(f.Lt), ptr(f.W), ptr(f.Wt),
+                  ptr(x0), float(tol_residual), float(tol_energy), max_iter, int(check_every), ptr(self._work),
+                  ptr(x), res)
+        return x, float(res[0]), float(res[1]), int(res[2])
+
+    def rayleigh_quotient(self, x):
+        """(x^T H x) / (x^T N x) in FP64 on the device."""
+        hx, nx = torch.empty_like(x), torch.empty_like(x)
+        self.H.apply_flat(x, hx)
+        self.N.apply_flat(x, nx)
+        out = empty((2,))
+        _lib.call("cc_dot_f64", stream(), self.n, ptr(x), ptr(hx), ptr(out[0:1]))
+        _lib.call("cc_dot_f64", stream(), self.n, ptr(x), ptr(nx), ptr(out[1:2]))
+        e, nn = out.tolist()
+        return e / nn
+
+
+# ---------------------------------------------------------------- relaxOver (reference-compatible)
+def _dot(x, y):
+    out = empty((1,))
+    _lib.call("cc_dot_f64", stream(), x.numel(), ptr(x), ptr(y), ptr(out))
+    return float(out.item())
+
+
+def _axpby(a, x, b, y):
+    _lib.call("cc_axpby_f64", stream(), x.numel(), float(a), ptr(x), float(b), ptr(y))
+
+
+class _DenseInverse:
+    """x -> N^-1 x through W = L^-1 of the formed matrix: N^-1 = W^T W."""
+
+    def __init__(self, matrix):
+        n = matrix.shape[0]
+        L = matrix.tensor().clone()
+        info = torch.zeros(1, dtype=torch.int32, device=L.device)
+        _lib.call("cc_cholesky_f64", stream(), n, ptr(L), ptr(info))
+        if int(info.item()) != 0:
+            raise ValueError("the normalization matrix is not positive definite (pivot {}); the HIP path "
+                             "factors N by Cholesky".format(int(info.item())))
+        self.W = empty(L.shape)
+        _lib.call("cc_trtri_lower_f64", stream(), n, ptr(L), ptr(self.W))
+        self.Wt = empty(L.shape)
+        _lib.call("cc_transpose_f64", stream(), n, ptr(self.W), ptr(self.Wt))
+        self.n = n
+
+    def __call__(self, x):
+        t = torch.empty_like(x)
+        y = torch.empty_like(x)
+        _lib.call("cc_skinny_f64", stream(), self.n, 1, ptr(self.W), 1, ptr(x), ptr(t))
+        _lib.call("cc_skinny_f64", stream(), self.n, 2, ptr(self.Wt), 1, ptr(t), ptr(y))
+        return y
+
+
+def relax_over(initial, expectation_multiplier, normalization_multiplier=None, maximum_number_of_multiplications=None,
+               tolerance=1e-7, dimension_of_krylov_space=None, stats=None):
+    """relaxOver (utils.py:805-878) with the vectors in device memory."""
+    shape = initial.shape
+    v = initial._t.reshape(-1).clone()
+    nv = np.sqrt(_dot(v, v))
+    _axpby(0.0, v, 1.0 / nv, v)
+    N = v.numel()
+    k = 3 if dimension_of_krylov_space is None else dimension_of_krylov_space
+
+    if normalization_multiplier is None:
+        inv_N = lambda x: x
+    else:
+        inv_N = _DenseInverse(normalization_multiplier.formMatrix())
+    if expectation_multiplier.isCheaperToFormMatrix(2 * k):
+        Hm = expectation_multiplier.formMatrix().tensor()
+
+        def mul_H(x):
+            y = torch.empty_like(x)
+            _lib.call("cc_skinny_f64", stream(), N, 0, ptr(Hm), 1, ptr(x), ptr(y))
+            return y
+    else:
+        op = getattr(expectation_multiplier, "operator", None)
+
+        def mul_H(x):
+            if op is not None:
+                y = torch.empty_like(x)
+                op.apply_flat(x, y)
+                return y
+            return expectation_multiplier(HipData(x.reshape(shape))).tensor().reshape(-1)
+    multiply = lambda x: inv_N(mul_H(x))
+    initial_value = _dot(v, multiply(v))
+    mults, last, complete, restarts = 0, None, (k == N), 0
+    while True:
+        basis = [v]
+        mbasis = []
+        for i in range(k):
+            mbasis.append(multiply(basis[i]))
+            if i < k - 1:
+                w = mbasis[i].clone()
+                coeffs = [_dot(b, mbasis[i]) for b in basis]     # classical Gram-Schmidt, one pass
+                for cf, b in zip(coeffs, basis):
+                    _axpby(-cf, b, 1.0, w)
+                nrm = np.sqrt(max(_dot(w, w), 0.0))
+                if nrm <= 1e-14:
+                    complete = True
+                    break
+                _axpby(0.0, w, 1.0 / nrm, w)
+                basis.append(w)
+        basis = basis[:len(mbasis)]
+        mults += k
+        restarts += 1
+        M = np.array([[_dot(b, mb) for mb in mbasis] for b in basis])
+        evals, evecs = eig(M)
+        j = int(np.argmin(evals.real))
+        lam, vec = evals[j], evecs[:, j]
+        stop = complete or (last is not None and abs(last - lam) <= tolerance) or \
+            (maximum_number_of_multiplications is not None and mults >= maximum_number_of_multiplications)
+        combo = torch.zeros_like(v)
+        for cf, b in zip(vec.real, basis):
+            _axpby(float(cf), b, 1.0, combo)
+        cn = np.sqrt(_dot(combo, combo))
+        _axpby(0.0, combo, 1.0 / cn, combo)
+        if stop:
+            final_value = _dot(combo, multiply(combo))
+            if stats is not None:
+                stats.update(restarts=restarts, mults=mults + 2, value=final_value)
+            denom = abs(final_value) + abs(initial_value)
+            if denom > 0 and (final_value - initial_value) / denom > 1 + 1e-7:
+                raise RelaxFailed(initial_value, final_value)
+            return HipData(combo.reshape(shape))
+        v = combo
+        last = lam

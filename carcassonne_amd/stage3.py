@@ -11,6 +11,8 @@ normalization contractor).  All terms of a solve are applied in one call to the
 HIP library: the fused MFMA kernel for the instantiated slab shapes, the generic
 GEMM path for any other shape.
 """
+import ctypes
+
 import numpy as np
 
 from . import _lib
@@ -61,6 +63,51 @@ class Stage3Operator:
                                     for a, b, op in zip(self._A, self._B, self._O))
         self.cost_of_formMatrix = sum(cost_of_matrix(a.shape, b.shape, self.d) for (a, b, _) in self.terms)
         self._work = None
+        self._fused = None
+        self.use_fused = (self.P == self.Q and
+                          bool(_lib.load().cc_stage3_fused_supported(self.P, self.d)) and
+                          self._plan_fits())
+
+    def _plan_fits(self):
+        na = len({id(a) for a in self._A})
+        nb = len({id(b) for b in self._B})
+        nops = len({id(o) for o in self._O if o is not None})
+        return na <= 16 and nb <= 16 and len(self._A) <= 48 and nops <= 16
+
+    def _build_fused(self):
+        """Pack every distinct slab channel once and build the term table (sorted by A channel)."""
+        a_ids, b_ids, o_ids = {}, {}, {}
+        a_list, b_list, ops = [], [], []
+        rows = []
+        for k, (A, B, O) in enumerate(zip(self._A, self._B, self._O)):
+            if id(A) not in a_ids:
+                a_ids[id(A)] = len(a_list)
+                a_list.append(A)
+            if id(B) not in b_ids:
+                b_ids[id(B)] = len(b_list)
+                b_list.append(B)
+            o = -1
+            if O is not None:
+                if id(O) not in o_ids:
+                    o_ids[id(O)] = len(ops)
+                    ops.append(O.toArray().reshape(-1))
+                o = o_ids[id(O)]
+            rows.append((a_ids[id(A)], k, b_ids[id(B)], o))
+        rows.sort()
+        size = _lib.query("cc_stage3_packed_size", self.n_alpha, self.P)
+        packs = []
+        for ch in a_list + b_list:
+            buf = empty((size,))
+            _lib.call("cc_stage3_pack_f64", stream(), self.n_alpha, self.P, ptr(ch._t), ptr(buf))
+            packs.append(buf)
+        self._fused = dict(
+            A=packs[:len(a_list)], B=packs[len(a_list):],
+            A_ptrs=_lib.ptr_array([t.data_ptr() for t in packs[:len(a_list)]]),
+            B_ptrs=_lib.ptr_array([t.data_ptr() for t in packs[len(a_list):]]),
+            ta=_lib.i32_array([r[0] for r in rows]), tb=_lib.i32_array([r[2] for r in rows]),
+            to=_lib.i32_array([r[3] for r in rows]), n_terms=len(rows), n_a=len(a_list), n_b=len(b_list),
+            n_ops=len(ops), ops=_lib.f64_array([x for o in ops for x in o]),
+            work=empty((max(1, _lib.query("cc_stage3_fused_work_size", self.n_alpha, self.P, self.d)),)))
 
     def _join(self, t, which):
         key = (id(t), which)
@@ -68,8 +115,43 @@ class Stage3Operator:
             self._joined[key] = t.join((0, 1), 4, 5, 2, 3) if which == 0 else t.join((1, 0), 4, 5, 2, 3)
         return self._joined[key]
 
+    def _ensure_fused(self):
+        if self._fused is None:
+            self._build_fused()
+
+    def plan_handle(self):
+        """A native plan (cc_stage3_plan_create) for solvers that apply this operator many times."""
+        self._ensure_fused()
+        f = self._fused
+        if f.get("plan") is None:
+            h = ctypes.c_void_p()
+            _lib.call("cc_stage3_plan_create", ctypes.byref(h), f["n_a"], f["A_ptrs"], f["n_b"], f["B_ptrs"],
+                      f["n_terms"], f["ta"], f["tb"], f["to"], f["n_ops"], f["ops"], self.n_alpha, self.P, self.d,
+                      ptr(f["work"]))
+            f["plan"] = h
+        return f["plan"]
+
+    def __del__(self):
+        f = getattr(self, "_fused", None)
+        if f and f.get("plan") is not None:
+            try:
+                _lib.load().cc_stage3_plan_destroy(f["plan"])
+            except Exception:
+                pass
+
     def apply_flat(self, x, y):
         """y = H x for flat device tensors of length n (torch float64)."""
+        if self.use_fused:
+            self._ensure_fused()
+            f = self._fused
+            _lib.call("cc_stage3_matvec_fused_f64", stream(), f["n_a"], f["A_ptrs"], f["n_b"], f["B_ptrs"],
+                      f["n_terms"], f["ta"], f["tb"], f["to"], f["n_ops"], f["ops"], self.n_alpha, self.P, self.d,
+                      ptr(x), ptr(y), ptr(f["work"]))
+            return
+        self.apply_generic_flat(x, y)
+
+    def apply_generic_flat(self, x, y):
+        """The reference's contraction order through the generic GEMM path (any shape)."""
         if self._work is None:
             ws = _lib.i64_array([0])
             _lib.call("cc_stage3_matvec_generic_work_size", self.n_alpha, self.P, self.Q, self.d, ws)

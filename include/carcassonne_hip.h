@@ -109,6 +109,57 @@ int cc_stage3_matvec_generic_f64(void* stream, int n_terms, const double* const*
                                  int64_t n_alpha, int64_t P, int64_t Q, int64_t d,
                                  const double* v, double* y, double* work);
 
+/* Fused FP64 MFMA stage-3 matvec for slab extents P = Q in {4, 9, 16, 25, 36} and d = 2
+ * (D = 2..6).  Slabs are first packed in MFMA-operand order with cc_stage3_pack_f64
+ * (cc_stage3_packed_size doubles per channel).  Terms (a, b, o) must be sorted by a;
+ * o indexes host_ops (n_ops row-major d x d matrices) or is -1 for the identity.
+ * `work` holds cc_stage3_fused_work_size doubles.  Same arithmetic as
+ * cc_stage3_matvec_generic_f64 (tensors/_2d/dense.py:115-173, sparse.py:129-133). */
+int cc_stage3_fused_supported(int64_t P, int64_t d);
+int cc_stage3_packed_size(int64_t n_alpha, int64_t P, int64_t* out);
+int cc_stage3_fused_work_size(int64_t n_alpha, int64_t P, int64_t d, int64_t* out);
+int cc_stage3_pack_f64(void* stream, int64_t n_alpha, int64_t P, const double* slabs, double* packed);
+int cc_stage3_matvec_fused_f64(void* stream, int n_a, const double* const* A_packed, int n_b,
+                               const double* const* B_packed, int n_terms, const int* term_a,
+                               const int* term_b, const int* term_o, int n_ops, const double* host_ops,
+                               int64_t n_alpha, int64_t P, int64_t d, const double* v, double* y,
+                               double* work);
+
+/* Persistent fused-operator plan: validated and copied once, applied many times (the
+ * operator of one solve).  `work` (cc_stage3_fused_work_size doubles) stays caller-owned.
+ * The plan object holding them is freed by cc_stage3_plan_destroy. */
+int cc_stage3_plan_create(void** out, int n_a, const double* const* A_packed, int n_b,
+                          const double* const* B_packed, int n_terms, const int* term_a,
+                          const int* term_b, const int* term_o, int n_ops, const double* host_ops,
+                          int64_t n_alpha, int64_t P, int64_t d, double* work);
+int cc_stage3_plan_apply(void* stream, const void* plan, const double* x, double* y);
+int cc_stage3_plan_destroy(void* plan);
+
+/* ---- dense linear algebra of the solve --------------------------------------------
+ * In-place blocked Cholesky A = L L^T (lower; upper zeroed); info_dev[0] = 0 or the first
+ * non-positive pivot (1-based).  The LU of the normalisation matrix in relaxOver
+ * (utils.py:814-817) for the symmetric positive definite case. */
+int cc_cholesky_f64(void* stream, int64_t n, double* A, int* info_dev);
+/* X = L^-1 for lower-triangular L (n x n). */
+int cc_trtri_lower_f64(void* stream, int64_t n, const double* L, double* X);
+/* B = A^T (n x n). */
+int cc_transpose_f64(void* stream, int64_t n, const double* A, double* B);
+/* Y (n x nrhs) = M (n x n) X restricted to tri = 0 full, 1 lower, 2 upper; nrhs <= 4. */
+int cc_skinny_f64(void* stream, int64_t n, int tri, const double* M, int nrhs, const double* X, double* Y);
+
+/* Lanczos with full reorthogonalisation for the lowest eigenpair of the pencil (H, N),
+ * N = (L L^T) (x) I_d, on the whitened operator W H W^T (W = L^-1).  H is a stage-3 plan.
+ * Stops when the Ritz residual <= tol_residual*|theta|, when the Ritz value moved by at most
+ * tol_energy*|theta| since the previous check (every check_every steps), or at max_iter.
+ * x_out is N-normalised; host_result = {theta, residual estimate, iterations}.
+ * Replaces relaxOver (utils.py:805-878) as the residual-controlled solver of
+ * minimizeExpectation (system/_2d.py:489-497). */
+int cc_lanczos_work_size(int64_t n, int max_iter, int64_t* out);
+int cc_lanczos_f64(void* stream, const void* plan, int64_t n_env, int64_t d, const double* L,
+                   const double* Lt, const double* W, const double* Wt, const double* x0,
+                   double tol_residual, double tol_energy, int max_iter, int check_every,
+                   double* work, double* x_out, double* host_result);
+
 #ifdef __cplusplus
 }
 #endif

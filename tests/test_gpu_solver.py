@@ -1,0 +1,93 @@
+"""GPU parity of the solve: dense linear algebra, the device Lanczos and the reference-compatible relaxOver."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def test_cholesky_trtri_skinny(gpu):
+    from carcassonne_amd import _lib
+    from carcassonne_amd.device import ptr, stream, empty
+    rng = np.random.default_rng(7)
+    for n in (5, 64, 65, 200, 333):
+        g = rng.standard_normal((n, n))
+        a = g @ g.T + n * np.eye(n)
+        A = torch.from_numpy(a.copy()).cuda()
+        info = torch.zeros(1, dtype=torch.int32, device=A.device)
+        _lib.call("cc_cholesky_f64", stream(), n, ptr(A), ptr(info))
+        assert int(info.item()) == 0
+        L = A.cpu().numpy()
+        np.testing.assert_allclose(L, np.linalg.cholesky(a), rtol=1e-11, atol=1e-11)
+        X = empty((n, n))
+        _lib.call("cc_trtri_lower_f64", stream(), n, ptr(A), ptr(X))
+        np.testing.assert_allclose(X.cpu().numpy() @ L, np.eye(n), atol=1e-11)
+        Ad = torch.from_numpy(a).cuda()      # keep device copies referenced until the kernels ran
+        for tri, m in ((0, a), (1, np.tril(a)), (2, np.triu(a))):
+            for nr in (1, 2, 3):
+                x = rng.standard_normal((n, nr))
+                xd = torch.from_numpy(x).cuda()
+                Y = empty((n, nr))
+                _lib.call("cc_skinny_f64", stream(), n, tri, ptr(Ad), nr, ptr(xd), ptr(Y))
+                np.testing.assert_allclose(Y.cpu().numpy(), m @ x, rtol=1e-12, atol=1e-10)
+    # a non positive definite matrix is reported, not factored silently
+    A = torch.from_numpy(-np.eye(8)).cuda()
+    info = torch.zeros(1, dtype=torch.int32, device=A.device)
+    _lib.call("cc_cholesky_f64", stream(), 8, ptr(A), ptr(info))
+    assert int(info.item()) == 1
+
+
+@pytest.mark.parametrize("D,chi,model,seed", [(2, 8, "ising", 0), (3, 6, "heisenberg", 1), (4, 8, "ising", 2),
+                                              (4, 16, "heisenberg", 3)])
+def test_lanczos_matches_generalized_eigh(gpu, D, chi, model, seed):
+    """Pinned against scipy.linalg.eigh(H, N) -- minimizeExpectationUsingFullEigensolver (system/_2d.py:498-502)."""
+    from scipy.linalg import eigh
+    from carcassonne_amd import synthetic as syn
+    from carcassonne_amd.tensors._2d import sparse as PS
+    from carcassonne_amd.solver import GroundStateSolver
+    from carcassonne_amd.data import HipData
+    from oracle import sparse2d
+    from tests.oracle_bridge import to_oracle, to_product
+    c, s, st = syn.make_environment(D, chi, model=model, seed=seed)
+    op = syn.operator_center(model)
+    ref = sparse2d.form_multipliers(*to_oracle(c, s, op))
+    w, V = eigh(ref.matrix_H(), ref.matrix_N())
+    H, N = PS.formExpectationAndNormalizationMultipliers(*to_product(c, s, op))
+    solver = GroundStateSolver(H.operator, N.operator)
+    x, theta, resid, iters = solver.solve(HipData.fromArray(st), tol_residual=1e-11, tol_energy=0.0, max_iter=600)
+    assert abs(theta - w[0]) <= 1e-10 * abs(w[0]), (theta, w[0], iters)
+    rq = solver.rayleigh_quotient(x)
+    assert abs(rq - w[0]) <= 1e-10 * abs(w[0])
+    xv = x.cpu().numpy()
+    Hm, Nm = ref.matrix_H(), ref.matrix_N()
+    assert abs(xv @ Nm @ xv - 1.0) < 1e-10          # N-normalised
+    r = Hm @ xv - theta * (Nm @ xv)
+    assert np.linalg.norm(r) <= 1e-8 * abs(theta) * np.linalg.norm(Nm @ xv)
+
+
+@pytest.mark.parametrize("D,chi,model,seed", [(2, 8, "ising", 3), (3, 5, "heisenberg", 4)])
+def test_relax_over_control_flow_matches_oracle(gpu, D, chi, model, seed):
+    """relaxOver on device vectors takes the same decisions as the oracle's restatement
+    (dense-vs-matrix-free, restarts, multiplications) and lands on the same value."""
+    from carcassonne_amd import synthetic as syn
+    from carcassonne_amd.tensors._2d import sparse as PS
+    from carcassonne_amd.solver import relax_over
+    from carcassonne_amd.data import HipData
+    from oracle import sparse2d, solver as OS
+    from tests.oracle_bridge import to_oracle, to_product
+    c, s, st = syn.make_environment(D, chi, model=model, seed=seed)
+    op = syn.operator_center(model)
+    ref = sparse2d.form_multipliers(*to_oracle(c, s, op))
+    use_lu = OS.is_cheaper_to_form_matrix(60, ref.cost_N, ref.cost_N_form, ref.n)
+    use_dense = OS.is_cheaper_to_form_matrix(6, ref.cost_H, ref.cost_H_form, ref.n)
+    ostats = {}
+    vo = OS.relax_over(st.ravel(), lambda x: ref.apply_H(x).ravel(), lambda x: ref.apply_N(x).ravel(), max_mults=100,
+                       tol=1e-7, H_matrix=ref.matrix_H() if use_dense else None, N_matrix=ref.matrix_N(),
+                       use_dense_H=use_dense, use_lu_N=True, stats=ostats)
+    H, N = PS.formExpectationAndNormalizationMultipliers(*to_product(c, s, op))
+    assert H.isCheaperToFormMatrix(6) == use_dense
+    pstats = {}
+    vp = relax_over(HipData.fromArray(st), H, N, maximum_number_of_multiplications=100, stats=pstats)
+    assert pstats["restarts"] == ostats["restarts"] and pstats["mults"] == ostats["mults"]
+    assert abs(pstats["value"] - ostats["value"].real) <= 1e-9 * abs(ostats["value"])
+    assert abs(abs(np.vdot(vp.toArray().ravel(), vo)) - 1.0) < 1e-6
