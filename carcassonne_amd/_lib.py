@@ -45,8 +45,11 @@ SIGNATURES = {
                               _i32, _pf64, _i64, _i64, _i64, _vp],
     "cc_stage3_plan_apply": [_vp, _vp, _vp, _vp],
     "cc_stage3_plan_destroy": [_vp],
-    "cc_cholesky_f64": [_vp, _i64, _vp, _vp],
-    "cc_trtri_lower_f64": [_vp, _i64, _vp, _vp],
+    "cc_profile_begin": [],
+    "cc_profile_end": [_pf64, _pi64],
+    "cc_linalg_work_size": [_i64, _pi64],
+    "cc_cholesky_f64": [_vp, _i64, _vp, _vp, _vp],
+    "cc_trtri_lower_f64": [_vp, _i64, _vp, _vp, _vp],
     "cc_transpose_f64": [_vp, _i64, _vp, _vp],
     "cc_skinny_f64": [_vp, _i64, _i32, _vp, _i32, _vp, _vp],
     "cc_lanczos_work_size": [_i64, _i32, _pi64],

@@ -24,13 +24,23 @@ __global__ void __launch_bounds__(256) lz_coeff_kernel(int64_t n, const double* 
   if (threadIdx.x == 0) c[blockIdx.x] = red[0];
 }
 
-// w[i] -= sum_j c[j] Q[j][i]   (or z[i] = sum_j c[j] Q[j][i] when `assign`)
-__global__ void lz_combine_kernel(int m, int64_t n, const double* __restrict__ Q, const double* __restrict__ c,
-                                  double* __restrict__ w, int assign) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    double acc = 0.0;
-    for (int j = 0; j < m; ++j) acc += c[j] * Q[(int64_t)j * n + i];
-    w[i] = assign ? acc : w[i] - acc;
+// w[i] -= sum_j c[j] Q[j][i]   (or z[i] = sum_j c[j] Q[j][i] when `assign`): block = 16 outputs x
+// 16 basis slices, slice sums combined in a fixed order.
+__global__ void __launch_bounds__(256) lz_combine_kernel(int m, int64_t n, const double* __restrict__ Q,
+                                                         const double* __restrict__ c, double* __restrict__ w, int assign) {
+  __shared__ double red[16][16];
+  const int e = threadIdx.x & 15, slice = threadIdx.x >> 4;
+  const int64_t i = (int64_t)blockIdx.x * 16 + e;
+  double acc = 0.0;
+  if (i < n)
+    for (int j = slice; j < m; j += 16) acc += c[j] * Q[(int64_t)j * n + i];
+  red[slice][e] = acc;
+  __syncthreads();
+  if (slice == 0 && i < n) {
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) t += red[k][e];
+    w[i] = assign ? t : w[i] - t;
   }
 }
 
@@ -61,6 +71,8 @@ __global__ void __launch_bounds__(1024) lz_normalize_kernel(int64_t n, const dou
   if (threadIdx.x == 0 && beta) *beta = b;
 }
 
+constexpr int LZ_LDS_M = 640;
+
 __device__ int sturm_count(int m, const double* a, const double* b, double x, double pivmin) {
   int cnt = 0;
   double q = a[0] - x;
@@ -76,10 +88,22 @@ __device__ int sturm_count(int m, const double* a, const double* b, double x, do
 
 // Lowest eigenpair of the m x m tridiagonal (a, b): multisection over one wave, then inverse
 // iteration (LU with partial pivoting) on lane 0.  out[0] = theta, out[1] = |b[m-1] * s[m-1]|.
-__global__ void __launch_bounds__(64) lz_tridiag_kernel(int m, const double* __restrict__ a, const double* __restrict__ b,
+__global__ void __launch_bounds__(64) lz_tridiag_kernel(int m, const double* __restrict__ ga, const double* __restrict__ gb,
                                                         double* __restrict__ out, double* __restrict__ s,
-                                                        double* __restrict__ scratch) {
+                                                        double* __restrict__ scratch, int want_vector) {
   const int lane = threadIdx.x;
+  // stage the tridiagonal in LDS (every lane reads all of it in the Sturm sequences)
+  __shared__ double lds_ab[2 * LZ_LDS_M];
+  const bool in_lds = m <= LZ_LDS_M;
+  if (in_lds) {
+    for (int i = lane; i < m; i += 64) {
+      lds_ab[i] = ga[i];
+      lds_ab[LZ_LDS_M + i] = gb[i];
+    }
+    __syncthreads();
+  }
+  const double* a = in_lds ? lds_ab : ga;
+  const double* b = in_lds ? lds_ab + LZ_LDS_M : gb;
   double lo = INFINITY, hi = -INFINITY, tnorm = 0.0;
   for (int i = lane; i < m; i += 64) {
     double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < m - 1 ? fabs(b[i]) : 0.0);
@@ -96,6 +120,8 @@ __global__ void __launch_bounds__(64) lz_tridiag_kernel(int m, const double* __r
   const double pivmin = fmax(tnorm * 1e-300, 1e-300) + 4.9e-324;
   lo -= 2.0 * eps * tnorm + pivmin;
   hi += 2.0 * eps * tnorm + pivmin;
+  // interlacing: the lowest Ritz value never increases with m, so the previous one bounds it
+  if (!isnan(out[0])) hi = fmin(hi, out[0] + 4.0 * eps * tnorm + pivmin);
   for (int round = 0; round < 20; ++round) {
     double x = lo + (hi - lo) * (double)(lane + 1) / 65.0;
     int cnt = sturm_count(m, a, b, x, pivmin);
@@ -113,14 +139,23 @@ __global__ void __launch_bounds__(64) lz_tridiag_kernel(int m, const double* __r
   }
   const double theta = 0.5 * (lo + hi);
   if (lane != 0) return;
-  double* d = scratch;
-  double* dl = scratch + m;
-  double* du = scratch + 2 * m;
-  double* du2 = scratch + 3 * m;
-  double* piv = scratch + 4 * m;
+  if (!want_vector) {
+    out[0] = theta;
+    out[1] = NAN;
+    return;
+  }
+  // LU scratch lives in LDS when it fits (the common case), else in global memory
+  __shared__ double lds_scr[6 * LZ_LDS_M];
+  double* base = (m <= LZ_LDS_M) ? lds_scr : scratch;
+  double* d = base;
+  double* dl = base + m;
+  double* du = base + 2 * m;
+  double* du2 = base + 3 * m;
+  double* piv = base + 4 * m;
   const double tiny = eps * fmax(tnorm, 1e-300);
-  for (int i = 0; i < m; ++i) s[i] = 1.0 + 0.01 * (double)((i * 7919) % 101) / 101.0;
-  for (int it = 0; it < 3; ++it) {
+  double* sv = (m <= LZ_LDS_M) ? base + 5 * m : s;  // the eigenvector, copied out at the end
+  for (int i = 0; i < m; ++i) sv[i] = 1.0 + 0.01 * (double)((i * 7919) % 101) / 101.0;
+  for (int it = 0; it < 2; ++it) {
     for (int i = 0; i < m; ++i) {
       d[i] = a[i] - theta;
       if (i < m - 1) { dl[i] = b[i]; du[i] = b[i]; }
@@ -150,24 +185,26 @@ __global__ void __launch_bounds__(64) lz_tridiag_kernel(int m, const double* __r
     if (d[m - 1] == 0.0) d[m - 1] = tiny;
     for (int i = 0; i < m - 1; ++i) {
       if (piv[i] == 0.0) {
-        s[i + 1] -= dl[i] * s[i];
+        sv[i + 1] -= dl[i] * sv[i];
       } else {
-        double t = s[i];
-        s[i] = s[i + 1];
-        s[i + 1] = t - dl[i] * s[i];
+        double t = sv[i];
+        sv[i] = sv[i + 1];
+        sv[i + 1] = t - dl[i] * sv[i];
       }
     }
-    s[m - 1] /= d[m - 1];
-    if (m > 1) s[m - 2] = (s[m - 2] - du[m - 2] * s[m - 1]) / d[m - 2];
-    for (int i = m - 3; i >= 0; --i) s[i] = (s[i] - du[i] * s[i + 1] - du2[i] * s[i + 2]) / d[i];
+    sv[m - 1] /= d[m - 1];
+    if (m > 1) sv[m - 2] = (sv[m - 2] - du[m - 2] * sv[m - 1]) / d[m - 2];
+    for (int i = m - 3; i >= 0; --i) sv[i] = (sv[i] - du[i] * sv[i + 1] - du2[i] * sv[i + 2]) / d[i];
     double nrm = 0.0, big = 0.0;
-    for (int i = 0; i < m; ++i) big = fmax(big, fabs(s[i]));
-    for (int i = 0; i < m; ++i) { s[i] /= big; nrm += s[i] * s[i]; }
+    for (int i = 0; i < m; ++i) big = fmax(big, fabs(sv[i]));
+    for (int i = 0; i < m; ++i) { sv[i] /= big; nrm += sv[i] * sv[i]; }
     nrm = sqrt(nrm);
-    for (int i = 0; i < m; ++i) s[i] /= nrm;
+    for (int i = 0; i < m; ++i) sv[i] /= nrm;
   }
+  if (sv != s)
+    for (int i = 0; i < m; ++i) s[i] = sv[i];
   out[0] = theta;
-  out[1] = fabs(b[m - 1] * s[m - 1]);
+  out[1] = fabs(b[m - 1] * sv[m - 1]);
 }
 
 struct Whitening {
@@ -217,6 +254,10 @@ extern "C" int cc_lanczos_f64(void* stream, const void* plan, int64_t n_env, int
   lz_normalize_kernel<<<1, 1024, 0, st>>>(n, w, Q, nullptr);
   CC_LAUNCH_CHECK();
 
+  {
+    const double nan2[2] = {NAN, NAN};
+    CC_HIP(hipMemcpyAsync(res, nan2, sizeof(nan2), hipMemcpyHostToDevice, st));
+  }
   double prev_theta = NAN, theta = NAN, resid = NAN;
   int m = 0;
   for (int j = 0; j < max_iter; ++j) {
@@ -230,28 +271,49 @@ extern "C" int cc_lanczos_f64(void* stream, const void* plan, int64_t n_env, int
     for (int pass = 0; pass < 2; ++pass) {
       lz_coeff_kernel<<<j + 1, 256, 0, st>>>(n, Q, w, c);
       CC_LAUNCH_CHECK();
-      lz_combine_kernel<<<gb, 256, 0, st>>>(j + 1, n, Q, c, w, 0);
+      lz_combine_kernel<<<(unsigned)((n + 15) / 16), 256, 0, st>>>(j + 1, n, Q, c, w, 0);
       CC_LAUNCH_CHECK();
     }
     lz_normalize_kernel<<<1, 1024, 0, st>>>(n, w, Q + (int64_t)(j + 1) * n, beta + j);
     CC_LAUNCH_CHECK();
     m = j + 1;
     if (m % check_every == 0 || m == max_iter || m == n) {
-      lz_tridiag_kernel<<<1, 64, 0, st>>>(m, alpha, beta, res, s, scratch);
-      CC_LAUNCH_CHECK();
-      double h[2];
-      CC_HIP(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, st));
-      CC_HIP(hipStreamSynchronize(st));
-      theta = h[0];
-      resid = h[1];
-      bool done = resid <= tol_residual * fabs(theta) || m == n ||
-                  (!std::isnan(prev_theta) && prev_theta - theta <= tol_energy * fabs(theta));
+      // Ritz value every check; the residual estimate (needs the Ritz vector) once the value has settled
+      bool settle_check = false;
+      for (int pass = 0; pass < 2; ++pass) {
+        const int want = pass == 1 ? 1 : 0;
+        lz_tridiag_kernel<<<1, 64, 0, st>>>(m, alpha, beta, res, s, scratch, want);
+        CC_LAUNCH_CHECK();
+        double h[2];
+        CC_HIP(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, st));
+        CC_HIP(hipStreamSynchronize(st));
+        theta = h[0];
+        if (pass == 0) {
+          settle_check = !std::isnan(prev_theta) && prev_theta - theta <= 1e3 * tol_energy * fabs(theta);
+          if (!settle_check && !(m == max_iter || m == n)) break;
+        } else {
+          resid = h[1];
+        }
+      }
+      bool done = m == n || m == max_iter ||
+                  (settle_check && (resid <= tol_residual * fabs(theta) ||
+                                    (prev_theta - theta <= tol_energy * fabs(theta))));
       prev_theta = theta;
       if (done) break;
     }
   }
+  // the final Ritz vector of T_m
+  lz_tridiag_kernel<<<1, 64, 0, st>>>(m, alpha, beta, res, s, scratch, 1);
+  CC_LAUNCH_CHECK();
+  {
+    double h[2];
+    CC_HIP(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, st));
+    CC_HIP(hipStreamSynchronize(st));
+    theta = h[0];
+    resid = h[1];
+  }
   // Ritz vector z = Q s, x = W^T z   (N-normalised because ||z|| = 1)
-  lz_combine_kernel<<<gb, 256, 0, st>>>(m, n, Q, s, t2, 1);
+  lz_combine_kernel<<<(unsigned)((n + 15) / 16), 256, 0, st>>>(m, n, Q, s, t2, 1);
   CC_LAUNCH_CHECK();
   rc = cc_skinny_f64(st, n_env, 2, Wt, (int)d, t2, x_out);
   if (rc) return rc;

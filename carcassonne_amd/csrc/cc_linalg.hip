@@ -6,56 +6,90 @@
 namespace {
 constexpr int CB = 64;  // Cholesky block size
 
-// Unblocked Cholesky of the kb x kb diagonal block at (k0, k0) in LDS; writes L (lower) back.
-__global__ void __launch_bounds__(256) chol_diag_kernel(int64_t n, int64_t k0, int kb, double* __restrict__ A,
-                                                        int* __restrict__ info) {
+__device__ inline double bcast(double v, int lane) {
+  // value of `v` in lane `lane` (a compile-time constant at every call site) through two readlanes
+  int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+// Cholesky of the kb x kb diagonal block at (k0, k0) on one wave: lane i holds row i in registers,
+// columns are broadcast with readlane.  The factor is staged in LDS and inverted (lane c solves
+// column c of L x = e_c with L rows read as LDS broadcasts).  Writes L back to A and L^-1 to dinv.
+__global__ void __launch_bounds__(64) chol_diag_kernel(int64_t n, int64_t k0, int kb, double* __restrict__ A,
+                                                       double* __restrict__ dinv, int* __restrict__ info) {
+  const int i = threadIdx.x;
+  double r[CB];
+#pragma unroll
+  for (int c = 0; c < CB; ++c) r[c] = (i < kb && c <= i) ? A[(k0 + i) * n + k0 + c] : (i == c ? 1.0 : 0.0);
+  int bad = 0;
+#pragma unroll
+  for (int j = 0; j < CB; ++j) {
+    double djj = bcast(r[j], j);
+    if (j < kb && !(djj > 0.0)) {
+      if (!bad) bad = j + 1;   // report the first non-positive pivot
+      djj = 1e-300;
+    }
+    const double sj = sqrt(djj);
+    r[j] = (i == j) ? sj : (i > j ? r[j] / sj : r[j]);
+#pragma unroll
+    for (int c = j + 1; c < CB; ++c) {
+      double acj = bcast(r[j], c);
+      if (c <= i) r[c] -= r[j] * acj;
+    }
+  }
+  if (i == 0 && bad && *info == 0) *info = (int)(k0 + bad);
+  // inverse: column `i` of X = L^-1 (row `row` of L is lane `row`'s registers)
+  double x[CB];
+#pragma unroll
+  for (int row = 0; row < CB; ++row) {
+    double acc = (row == i) ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < row; ++k) acc -= bcast(r[k], row) * x[k];
+    x[row] = (row < i) ? 0.0 : acc / bcast(r[row], row);
+  }
+  if (i < kb) {
+#pragma unroll
+    for (int c = 0; c < CB; ++c)
+      if (c < kb) A[(k0 + i) * n + k0 + c] = (c <= i) ? r[c] : 0.0;
+  }
+  // dinv[row][i] = x[row]
+#pragma unroll
+  for (int row = 0; row < CB; ++row)
+    if (row < kb && i < kb) dinv[row * kb + i] = x[row];
+}
+
+// Inverse of every diagonal block of L into X (block g on workgroup g).
+__global__ void __launch_bounds__(256) diag_block_inverse_kernel(int64_t n, const double* __restrict__ L,
+                                                                 double* __restrict__ X) {
   __shared__ double a[CB][CB + 1];
+  __shared__ double x[CB][CB + 1];
+  const int64_t k0 = (int64_t)blockIdx.x * CB;
+  const int kb = (int)((n - k0) < CB ? (n - k0) : CB);
   for (int e = threadIdx.x; e < kb * kb; e += 256) {
     int i = e / kb, j = e % kb;
-    a[i][j] = (j <= i) ? A[(k0 + i) * n + k0 + j] : 0.0;
+    a[i][j] = L[(k0 + i) * n + k0 + j];
   }
   __syncthreads();
-  for (int j = 0; j < kb; ++j) {
-    __syncthreads();
-    double d = a[j][j];
-    if (!(d > 0.0)) {
-      if (threadIdx.x == 0 && *info == 0) *info = (int)(k0 + j + 1);
-      d = 1e-300;
-    }
-    double s = sqrt(d);
-    __syncthreads();
-    if (threadIdx.x == 0) a[j][j] = s;
-    for (int i = j + 1 + threadIdx.x; i < kb; i += 256) a[i][j] /= s;
-    __syncthreads();
-    for (int e = threadIdx.x; e < (kb - j - 1) * (kb - j - 1); e += 256) {
-      int i = j + 1 + e / (kb - j - 1), c = j + 1 + e % (kb - j - 1);
-      if (c <= i) a[i][c] -= a[i][j] * a[c][j];
+  for (int c = threadIdx.x; c < kb; c += 256) {
+    for (int i = 0; i < kb; ++i) {
+      double v = (i == c) ? 1.0 : 0.0;
+      for (int k = c; k < i; ++k) v -= a[i][k] * x[k][c];
+      x[i][c] = (i < c) ? 0.0 : v / a[i][i];
     }
   }
   __syncthreads();
   for (int e = threadIdx.x; e < kb * kb; e += 256) {
     int i = e / kb, j = e % kb;
-    A[(k0 + i) * n + k0 + j] = (j <= i) ? a[i][j] : 0.0;
+    X[(k0 + i) * n + k0 + j] = x[i][j];
   }
 }
 
-// Panel: rows i >= k0+kb, columns [k0, k0+kb): solve X L_kk^T = A_panel, one thread per row.
-__global__ void __launch_bounds__(256) chol_panel_kernel(int64_t n, int64_t k0, int kb, double* __restrict__ A) {
-  __shared__ double l[CB][CB + 1];
-  for (int e = threadIdx.x; e < kb * kb; e += 256) {
-    int i = e / kb, j = e % kb;
-    l[i][j] = A[(k0 + i) * n + k0 + j];
-  }
-  __syncthreads();
-  int64_t row = k0 + kb + (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (row >= n) return;
-  double x[CB];
-  for (int j = 0; j < kb; ++j) {
-    double v = A[row * n + k0 + j];
-    for (int c = 0; c < j; ++c) v -= x[c] * l[j][c];
-    x[j] = v / l[j][j];
-  }
-  for (int j = 0; j < kb; ++j) A[row * n + k0 + j] = x[j];
+// dst[r][c] = src[r][c] for an rows x cols block with leading dimensions lds / ldd.
+__global__ void copy2d_kernel(int64_t rows, int64_t cols, const double* __restrict__ src, int64_t lds,
+                              double* __restrict__ dst, int64_t ldd) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < rows * cols; e += (int64_t)gridDim.x * blockDim.x)
+    dst[(e / cols) * ldd + e % cols] = src[(e / cols) * lds + e % cols];
 }
 
 __global__ void set_int_kernel(int* p, int v) { *p = v; }
@@ -64,31 +98,6 @@ __global__ void set_int_kernel(int* p, int v) { *p = v; }
 __global__ void zero_upper_kernel(int64_t n, double* __restrict__ A) {
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += (int64_t)gridDim.x * blockDim.x)
     if (e % n > e / n) A[e] = 0.0;
-}
-
-// X = L^-1 by forward substitution on the identity; workgroup g owns columns [32g, 32g+32).
-// 256 threads = 32 columns x 8 partial sums per column (fixed-order reduction).
-__global__ void __launch_bounds__(256) trtri_lower_kernel(int64_t n, const double* __restrict__ L, double* __restrict__ X) {
-  const int cl = threadIdx.x & 31, part = threadIdx.x >> 5;
-  const int64_t c = (int64_t)blockIdx.x * 32 + cl;
-  const int64_t c0 = (int64_t)blockIdx.x * 32;
-  __shared__ double red[8][32];
-  // zero the rows above the block and the block's own upper part
-  for (int64_t i = 0; i < n; ++i) {
-    double acc = 0.0;
-    if (c < n && i >= c0) {
-      for (int64_t k = c0 + part; k < i; k += 8) acc += L[i * n + k] * X[k * n + c];
-    }
-    red[part][cl] = acc;
-    __syncthreads();
-    if (part == 0 && c < n) {
-      double s = 0.0;
-      for (int p = 0; p < 8; ++p) s += red[p][cl];
-      double rhs = (i == c) ? 1.0 : 0.0;
-      X[i * n + c] = (i < c) ? 0.0 : (rhs - s) / L[i * n + i];
-    }
-    __syncthreads();
-  }
 }
 
 // Y[i][s] = sum_k M[i][k] X[k][s] for k in [kbeg(i), kend(i)) with M row-major n x n,
@@ -130,22 +139,34 @@ __global__ void transpose_kernel(int64_t n, const double* __restrict__ A, double
 }
 }  // namespace
 
-extern "C" int cc_cholesky_f64(void* stream, int64_t n, double* A, int* info_dev) {
+extern "C" int cc_linalg_work_size(int64_t n, int64_t* out) {
+  *out = (int64_t)CB * CB + (int64_t)CB * n;
+  return CC_OK;
+}
+
+// Right-looking blocked Cholesky: diagonal block in LDS (with its inverse), the panel as a GEMM
+// against that inverse, the trailing update as a GEMM.
+extern "C" int cc_cholesky_f64(void* stream, int64_t n, double* A, int* info_dev, double* work) {
   if (n <= 0) { cc_set_last_error("cholesky: n must be positive"); return CC_ERR_ARG; }
   hipStream_t st = cc_stream(stream);
+  double* dinv = work;
+  double* tmp = work + CB * CB;
   set_int_kernel<<<1, 1, 0, st>>>(info_dev, 0);
   CC_LAUNCH_CHECK();
   for (int64_t k0 = 0; k0 < n; k0 += CB) {
     int kb = (int)((n - k0) < CB ? (n - k0) : CB);
-    chol_diag_kernel<<<1, 256, 0, st>>>(n, k0, kb, A, info_dev);
+    chol_diag_kernel<<<1, 64, 0, st>>>(n, k0, kb, A, dinv, info_dev);
     CC_LAUNCH_CHECK();
     int64_t rest = n - k0 - kb;
     if (rest > 0) {
-      chol_panel_kernel<<<(unsigned)((rest + 255) / 256), 256, 0, st>>>(n, k0, kb, A);
+      double* Pn = A + (k0 + kb) * n + k0;
+      // tmp = A_panel * Linv_kk^T ; A_panel = tmp
+      int rc = cc_gemm_f64(stream, 1, rest, kb, kb, 1.0, Pn, 0, n, 1, dinv, 0, 1, kb, 0.0, tmp, 0, kb, 1);
+      if (rc) return rc;
+      copy2d_kernel<<<cc_blocks(rest * kb, 256), 256, 0, st>>>(rest, kb, tmp, kb, Pn, n);
       CC_LAUNCH_CHECK();
-      const double* Lp = A + (k0 + kb) * n + k0;
       double* T = A + (k0 + kb) * n + (k0 + kb);
-      int rc = cc_gemm_f64(stream, 1, rest, rest, kb, -1.0, Lp, 0, n, 1, Lp, 0, 1, n, 1.0, T, 0, n, 1);
+      rc = cc_gemm_f64(stream, 1, rest, rest, kb, -1.0, Pn, 0, n, 1, Pn, 0, 1, n, 1.0, T, 0, n, 1);
       if (rc) return rc;
     }
   }
@@ -154,10 +175,23 @@ extern "C" int cc_cholesky_f64(void* stream, int64_t n, double* A, int* info_dev
   return CC_OK;
 }
 
-extern "C" int cc_trtri_lower_f64(void* stream, int64_t n, const double* L, double* X) {
+// X = L^-1, blocked: diagonal-block inverses, then block row i: X[i, :i] = -X_ii (L[i, :i] X[:i, :i]).
+extern "C" int cc_trtri_lower_f64(void* stream, int64_t n, const double* L, double* X, double* work) {
   if (n <= 0) { cc_set_last_error("trtri: n must be positive"); return CC_ERR_ARG; }
-  trtri_lower_kernel<<<(unsigned)((n + 31) / 32), 256, 0, cc_stream(stream)>>>(n, L, X);
+  hipStream_t st = cc_stream(stream);
+  int rc = cc_fill_f64(stream, n * n, 0.0, X);
+  if (rc) return rc;
+  const int64_t nblk = (n + CB - 1) / CB;
+  diag_block_inverse_kernel<<<(unsigned)nblk, 256, 0, st>>>(n, L, X);
   CC_LAUNCH_CHECK();
+  double* S = work;  // CB x n
+  for (int64_t i = 1; i < nblk; ++i) {
+    int64_t r0 = i * CB, rb = (n - r0) < CB ? (n - r0) : CB, c = r0;
+    rc = cc_gemm_f64(stream, 1, rb, c, c, 1.0, L + r0 * n, 0, n, 1, X, 0, n, 1, 0.0, S, 0, c, 1);
+    if (rc) return rc;
+    rc = cc_gemm_f64(stream, 1, rb, c, rb, -1.0, X + r0 * n + r0, 0, n, 1, S, 0, c, 1, 0.0, X + r0 * n, 0, n, 1);
+    if (rc) return rc;
+  }
   return CC_OK;
 }
 

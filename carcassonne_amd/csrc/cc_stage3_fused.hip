@@ -14,7 +14,13 @@
 // writes one partial; a second kernel sums the partials in a fixed order.
 #include "cc_common.h"
 
+#include <vector>
+
 namespace {
+// Optional timing of every fused-kernel launch (HIP events on the launch stream).
+bool g_profile = false;
+std::vector<hipEvent_t> g_prof_events;  // start/stop pairs
+
 constexpr int MAX_CH = 16, MAX_TERMS = 48, MAX_OPS = 16, MAX_D = 4;
 
 struct FusedPlan {
@@ -158,20 +164,43 @@ __global__ void __launch_bounds__(256) fused_f64_kernel(const FusedPlan plan, co
     partial[(int64_t)blockIdx.x * NY + i] = ((red0[i] + red0[NY + i]) + red0[2 * NY + i]) + red0[3 * NY + i];
 }
 
-__global__ void reduce_partials_kernel(int64_t groups, int64_t n, const double* __restrict__ partial, double* __restrict__ y) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    double acc = 0.0;
-    for (int64_t g = 0; g < groups; ++g) acc += partial[g * n + i];
-    y[i] = acc;
+// y[i] = sum_g partial[g][i] in a fixed order: block = 16 consecutive outputs x 16 group slices;
+// slice k sums groups k, k+16, ...; the sixteen slice sums are combined in order 0..15.
+__global__ void __launch_bounds__(256) reduce_partials_kernel(int64_t groups, int64_t n, const double* __restrict__ partial,
+                                                              double* __restrict__ y) {
+  __shared__ double red[16][16];
+  const int e = threadIdx.x & 15, slice = threadIdx.x >> 4;
+  const int64_t i = (int64_t)blockIdx.x * 16 + e;
+  double acc = 0.0;
+  if (i < n)
+    for (int64_t g = slice; g < groups; g += 16) acc += partial[g * n + i];
+  red[slice][e] = acc;
+  __syncthreads();
+  if (slice == 0 && i < n) {
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) t += red[k][e];
+    y[i] = t;
   }
 }
 
 template <int P, int DP>
 int launch_fused(hipStream_t st, const FusedPlan& plan, const double* v, double* y, double* work, int groups) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (g_profile) {
+    CC_HIP(hipEventCreate(&e0));
+    CC_HIP(hipEventCreate(&e1));
+    CC_HIP(hipEventRecord(e0, st));
+  }
   fused_f64_kernel<P, DP><<<groups, 256, 0, st>>>(plan, v, work);
   CC_LAUNCH_CHECK();
+  if (g_profile) {
+    CC_HIP(hipEventRecord(e1, st));
+    g_prof_events.push_back(e0);
+    g_prof_events.push_back(e1);
+  }
   const int64_t n = (int64_t)P * P * DP;
-  reduce_partials_kernel<<<cc_blocks(n, 256), 256, 0, st>>>(groups, n, work, y);
+  reduce_partials_kernel<<<(unsigned)((n + 15) / 16), 256, 0, st>>>(groups, n, work, y);
   CC_LAUNCH_CHECK();
   return CC_OK;
 }
@@ -313,5 +342,28 @@ extern "C" int cc_stage3_plan_apply(void* stream, const void* plan, const double
 
 extern "C" int cc_stage3_plan_destroy(void* plan) {
   delete static_cast<Stage3PlanHandle*>(plan);
+  return CC_OK;
+}
+
+extern "C" int cc_profile_begin(void) {
+  for (hipEvent_t e : g_prof_events) (void)hipEventDestroy(e);
+  g_prof_events.clear();
+  g_profile = true;
+  return CC_OK;
+}
+
+extern "C" int cc_profile_end(double* total_ms, int64_t* launches) {
+  g_profile = false;
+  double total = 0.0;
+  for (size_t i = 0; i + 1 < g_prof_events.size(); i += 2) {
+    CC_HIP(hipEventSynchronize(g_prof_events[i + 1]));
+    float ms = 0.0f;
+    CC_HIP(hipEventElapsedTime(&ms, g_prof_events[i], g_prof_events[i + 1]));
+    total += ms;
+  }
+  *total_ms = total;
+  *launches = (int64_t)(g_prof_events.size() / 2);
+  for (hipEvent_t e : g_prof_events) (void)hipEventDestroy(e);
+  g_prof_events.clear();
   return CC_OK;
 }

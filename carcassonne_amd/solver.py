@@ -45,14 +45,15 @@ class NormalizationFactor:
         self.N_env = Nenv
         L = Nenv.clone()
         info = torch.zeros(1, dtype=torch.int32, device=L.device)
-        _lib.call("cc_cholesky_f64", stream(), self.n_env, ptr(L), ptr(info))
+        work = empty((_lib.query("cc_linalg_work_size", self.n_env),))
+        _lib.call("cc_cholesky_f64", stream(), self.n_env, ptr(L), ptr(info), ptr(work))
         if int(info.item()) != 0:
             raise ValueError("the normalization matrix is not positive definite (pivot {})".format(int(info.item())))
         self.L = L
         self.Lt = empty(L.shape)
         _lib.call("cc_transpose_f64", stream(), self.n_env, ptr(L), ptr(self.Lt))
         self.W = empty(L.shape)
-        _lib.call("cc_trtri_lower_f64", stream(), self.n_env, ptr(L), ptr(self.W))
+        _lib.call("cc_trtri_lower_f64", stream(), self.n_env, ptr(L), ptr(self.W), ptr(work))
         self.Wt = empty(L.shape)
         _lib.call("cc_transpose_f64", stream(), self.n_env, ptr(self.W), ptr(self.Wt))
 
@@ -118,12 +119,13 @@ class _DenseInverse:
         n = matrix.shape[0]
         L = matrix.tensor().clone()
         info = torch.zeros(1, dtype=torch.int32, device=L.device)
-        _lib.call("cc_cholesky_f64", stream(), n, ptr(L), ptr(info))
+        work = empty((_lib.query("cc_linalg_work_size", n),))
+        _lib.call("cc_cholesky_f64", stream(), n, ptr(L), ptr(info), ptr(work))
         if int(info.item()) != 0:
             raise ValueError("the normalization matrix is not positive definite (pivot {}); the HIP path "
                              "factors N by Cholesky".format(int(info.item())))
         self.W = empty(L.shape)
-        _lib.call("cc_trtri_lower_f64", stream(), n, ptr(L), ptr(self.W))
+        _lib.call("cc_trtri_lower_f64", stream(), n, ptr(L), ptr(self.W), ptr(work))
         self.Wt = empty(L.shape)
         _lib.call("cc_transpose_f64", stream(), n, ptr(self.W), ptr(self.Wt))
         self.n = n

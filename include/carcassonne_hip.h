@@ -135,13 +135,20 @@ int cc_stage3_plan_create(void** out, int n_a, const double* const* A_packed, in
 int cc_stage3_plan_apply(void* stream, const void* plan, const double* x, double* y);
 int cc_stage3_plan_destroy(void* plan);
 
+/* Timing of the fused stage-3 kernel: between begin and end every launch is bracketed by
+ * HIP events on its stream; end returns the summed kernel time and the launch count.
+ * Single-threaded use only.  (Measurement support; no reference counterpart.) */
+int cc_profile_begin(void);
+int cc_profile_end(double* total_ms, int64_t* launches);
+
 /* ---- dense linear algebra of the solve --------------------------------------------
  * In-place blocked Cholesky A = L L^T (lower; upper zeroed); info_dev[0] = 0 or the first
- * non-positive pivot (1-based).  The LU of the normalisation matrix in relaxOver
+ * non-positive pivot (1-based); `work` holds cc_linalg_work_size doubles.  The LU of the normalisation matrix in relaxOver
  * (utils.py:814-817) for the symmetric positive definite case. */
-int cc_cholesky_f64(void* stream, int64_t n, double* A, int* info_dev);
-/* X = L^-1 for lower-triangular L (n x n). */
-int cc_trtri_lower_f64(void* stream, int64_t n, const double* L, double* X);
+int cc_linalg_work_size(int64_t n, int64_t* out);
+int cc_cholesky_f64(void* stream, int64_t n, double* A, int* info_dev, double* work);
+/* X = L^-1 for lower-triangular L (n x n); `work` holds cc_linalg_work_size doubles. */
+int cc_trtri_lower_f64(void* stream, int64_t n, const double* L, double* X, double* work);
 /* B = A^T (n x n). */
 int cc_transpose_f64(void* stream, int64_t n, const double* A, double* B);
 /* Y (n x nrhs) = M (n x n) X restricted to tri = 0 full, 1 lower, 2 upper; nrhs <= 4. */

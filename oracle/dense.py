@@ -70,13 +70,14 @@ def absorb_center_sos_into_side(direction, T, S, Op, Sc=None):
 
 def stage1(C, T):
     """dense.py:96-99: C(3,4,5)<->T(0,1,2) -> ((C0,C1,C2),(T3,T4,T5),T6,T7)."""
-    r = np.einsum("abcdef,defghijk->abcghijk", C, T)
+    r = np.tensordot(C, T, ((3, 4, 5), (0, 1, 2)))            # (C0,C1,C2,T3..T7)
     return _merge(r, 3, 3, 1, 1)
 
 
 def stage2(P, Q):
     """dense.py:102-112: P0<->Q1 -> (Q0,P1,P2,Q2,P3,Q3)."""
-    return np.einsum("xbcd,axef->abcedf", P, Q)
+    r = np.tensordot(P, Q, ((0,), (1,)))                       # (P1,P2,P3,Q0,Q2,Q3)
+    return r.transpose(3, 0, 1, 4, 2, 5)
 
 
 def join_a(s2_0):
@@ -96,22 +97,26 @@ def stage3_apply(A, B, v, Op=None):
     (dense.py:146-173): Y = sum_alpha A_alpha (x) B_alpha (x) Op applied to v, with A, B
     in joined slab layout.  Op=None is the normalization contractor (no physical op)."""
     if Op is not None:
-        v = np.einsum("st,abcdt->abcds", Op, v)
-    t = np.einsum("zpqab,abcds->zpqcds", A, v)
-    return np.einsum("zrscd,zpqcdt->pqrst", B, t)
+        v = np.tensordot(v, Op, (4, 1))                        # (a,b,c,d,s') ; O1 <-> v4
+    t = np.tensordot(A, v, ((3, 4), (0, 1)))                    # (z,p,q,c,d,s)
+    r = np.tensordot(B, t, ((0, 3, 4), (0, 3, 4)))              # (r,s2,p,q,s)
+    return r.transpose(2, 3, 0, 1, 4)
 
 
 def stage3_matrix(s2_0, s2_1, Op):
     """formDenseStage3_formMatrix (dense.py:176-194): rows (s2_0 4,5; s2_1 4,5; Op0),
     cols (s2_0 2,3; s2_1 2,3; Op1)."""
-    m = np.einsum("xyabcd,yxefgh,st->cdghsabeft", s2_0, s2_1, Op)
+    e = np.tensordot(s2_0, s2_1, ((0, 1), (1, 0)))             # (a,b,c,d,e,f,g,h)
+    m = np.multiply.outer(e, Op)                                 # (a,b,c,d,e,f,g,h,s,t)
+    m = m.transpose(2, 3, 6, 7, 8, 0, 1, 4, 5, 9)               # rows (c,d,g,h,s), cols (a,b,e,f,t)
     n = int(np.prod(m.shape[:5]))
     return m.reshape(n, n)
 
 
 def normalization_submatrix(s2_0, s2_1):
     """formNormalizationSubmatrix (dense.py:205-224)."""
-    m = np.einsum("xyabcd,yxefgh->cdghabef", s2_0, s2_1)
+    e = np.tensordot(s2_0, s2_1, ((0, 1), (1, 0)))
+    m = e.transpose(2, 3, 6, 7, 0, 1, 4, 5)
     n = int(np.prod(m.shape[:4]))
     return m.reshape(n, n)
 
@@ -135,3 +140,13 @@ def cost_stage3_matrix(s2_0_shape, s2_1_shape, d):
     _, _, k2, k3, b2, b3 = s2_1_shape
     m = k0 * k1 * b0 * b1 * k2 * k3 * b2 * b3
     return int(m * x * y + m * d * d)
+
+
+def normalization_env_matrix(AN, BN):
+    """N_env[(p',q'),(p,q)] = sum_alpha A[alpha][p'][p] B[alpha][q'][q] for joined identity slabs;
+    the full normalization matrix is N_env (x) I_d."""
+    na, a1, a2, k0, k1 = AN.shape
+    _, b1, b2, k2, k3 = BN.shape
+    t = np.tensordot(AN, BN, ((0,), (0,)))                     # (a1,a2,k0,k1,b1,b2,k2,k3)
+    t = t.transpose(0, 1, 4, 5, 2, 3, 6, 7)
+    return t.reshape(a1 * a2 * b1 * b2, k0 * k1 * k2 * k3)

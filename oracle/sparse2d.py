@@ -162,7 +162,7 @@ class Stage3Operator:
         return m
 
     def matrix_N(self):
-        return dense.stage3_matrix(self.s2[0][Identity()], self.s2[1][Identity()], self.eye)
+        return np.kron(dense.normalization_env_matrix(self.AN, self.BN), self.eye)
 
 
 def form_multipliers(corners, sides, op_center):

@@ -10,17 +10,18 @@ def test_cholesky_trtri_skinny(gpu):
     from carcassonne_amd import _lib
     from carcassonne_amd.device import ptr, stream, empty
     rng = np.random.default_rng(7)
-    for n in (5, 64, 65, 200, 333):
+    for n in (5, 64, 65, 200, 333, 1296):
         g = rng.standard_normal((n, n))
         a = g @ g.T + n * np.eye(n)
         A = torch.from_numpy(a.copy()).cuda()
         info = torch.zeros(1, dtype=torch.int32, device=A.device)
-        _lib.call("cc_cholesky_f64", stream(), n, ptr(A), ptr(info))
+        work = empty((_lib.query("cc_linalg_work_size", n),))
+        _lib.call("cc_cholesky_f64", stream(), n, ptr(A), ptr(info), ptr(work))
         assert int(info.item()) == 0
         L = A.cpu().numpy()
         np.testing.assert_allclose(L, np.linalg.cholesky(a), rtol=1e-11, atol=1e-11)
         X = empty((n, n))
-        _lib.call("cc_trtri_lower_f64", stream(), n, ptr(A), ptr(X))
+        _lib.call("cc_trtri_lower_f64", stream(), n, ptr(A), ptr(X), ptr(work))
         np.testing.assert_allclose(X.cpu().numpy() @ L, np.eye(n), atol=1e-11)
         Ad = torch.from_numpy(a).cuda()      # keep device copies referenced until the kernels ran
         for tri, m in ((0, a), (1, np.tril(a)), (2, np.triu(a))):
@@ -33,7 +34,8 @@ def test_cholesky_trtri_skinny(gpu):
     # a non positive definite matrix is reported, not factored silently
     A = torch.from_numpy(-np.eye(8)).cuda()
     info = torch.zeros(1, dtype=torch.int32, device=A.device)
-    _lib.call("cc_cholesky_f64", stream(), 8, ptr(A), ptr(info))
+    work = empty((_lib.query("cc_linalg_work_size", 8),))
+    _lib.call("cc_cholesky_f64", stream(), 8, ptr(A), ptr(info), ptr(work))
     assert int(info.item()) == 1
 
 
