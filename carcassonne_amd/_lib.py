@@ -54,6 +54,18 @@ SIGNATURES = {
     "cc_skinny_f64": [_vp, _i64, _i32, _vp, _i32, _vp, _vp],
     "cc_lanczos_work_size": [_i64, _i32, _pi64],
     "cc_lanczos_f64": [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _f64, _f64, _i32, _i32, _vp, _vp, _pf64],
+    "cc_stage3_lp_convert": [_vp, _i32, _i64, _vp, _vp, _vp],
+    "cc_stage3_lp_work_size": [_i64, _i64, _i64, _i32, _pi64],
+    "cc_stage3_lp_plan_create": [ctypes.POINTER(ctypes.c_void_p), _i32, _i32, _pvp, _i32, _pvp, _i32, _pi32, _pi32,
+                                 ctypes.POINTER(ctypes.c_float), _i64, _i64, _i64, _i32, _i32, _vp],
+    "cc_stage3_lp_plan_apply": [_vp, _vp, _vp, _vp],
+    "cc_stage3_lp_plan_destroy": [_vp],
+    "cc_lanczos_lp": [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _f64, _f64, _i32, _i32, _vp, _vp, _pf64],
+    "cc_basis_dots_f64": [_vp, _i32, _i64, _vp, _vp, _vp, _i32],
+    "cc_basis_combine_f64": [_vp, _i32, _i64, _vp, _vp, _vp],
+    "cc_small_geneig_f64": [_vp, _i32, _vp, _vp, _vp, _vp],
+    "cc_jd_work_size": [_i64, _pi64],
+    "cc_jd_correction_lp": [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _f64, _vp, _vp, _i32, _vp, _vp],
     "cc_stage3_matvec_fused_f64": [_vp, _i32, _pvp, _i32, _pvp, _i32, _pi32, _pi32, _pi32, _i32, _pf64, _i64, _i64,
                                    _i64, _vp, _vp, _vp],
 }
@@ -122,6 +134,11 @@ def i32_array(values):
 def f64_array(values):
     values = [float(v) for v in values]
     return (ctypes.c_double * max(1, len(values)))(*values)
+
+
+def f32_array(values):
+    values = [float(v) for v in values]
+    return (ctypes.c_float * max(1, len(values)))(*values)
 
 
 def query(name, *args):

@@ -42,3 +42,5 @@ static inline unsigned cc_blocks(int64_t n, int threads, unsigned cap = 65535u *
 // Internal (not exported through the C ABI header): operator plans used by the solvers.
 int cc_internal_stage3_apply(hipStream_t st, const void* plan, const double* x, double* y);
 int64_t cc_internal_stage3_n(const void* plan);
+int cc_internal_lp_apply(hipStream_t st, const void* plan, const double* x, double* y);
+int64_t cc_internal_lp_n(const void* plan);

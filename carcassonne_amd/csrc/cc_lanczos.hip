@@ -212,10 +212,13 @@ struct Whitening {
   const double *L, *Lt, *W, *Wt;
 };
 
-int op_whitened(hipStream_t st, const void* plan, const Whitening& wh, const double* q, double* t1, double* t2, double* w) {
+typedef int (*ApplyFn)(hipStream_t, const void*, const double*, double*);
+
+int op_whitened(hipStream_t st, ApplyFn apply, const void* plan, const Whitening& wh, const double* q, double* t1, double* t2,
+                double* w) {
   int rc = cc_skinny_f64(st, wh.n_env, 2, wh.Wt, (int)wh.d, q, t1);  // t1 = W^T q
   if (rc) return rc;
-  rc = cc_internal_stage3_apply(st, plan, t1, t2);                   // t2 = H t1
+  rc = apply(st, plan, t1, t2);                                      // t2 = H t1
   if (rc) return rc;
   return cc_skinny_f64(st, wh.n_env, 1, wh.W, (int)wh.d, t2, w);     // w = W t2
 }
@@ -226,14 +229,35 @@ extern "C" int cc_lanczos_work_size(int64_t n, int max_iter, int64_t* out) {
   return CC_OK;
 }
 
+static int lanczos_driver(hipStream_t st, ApplyFn apply, const void* plan, int64_t n_env, int64_t d, const double* L,
+                          const double* Lt, const double* W, const double* Wt, const double* x0, double tol_residual,
+                          double tol_energy, int max_iter, int check_every, double* work, double* x_out,
+                          double* host_result);
+
 extern "C" int cc_lanczos_f64(void* stream, const void* plan, int64_t n_env, int64_t d, const double* L, const double* Lt,
                               const double* W, const double* Wt, const double* x0, double tol_residual,
                               double tol_energy, int max_iter, int check_every, double* work, double* x_out,
                               double* host_result /* theta, residual, iterations */) {
-  hipStream_t st = cc_stream(stream);
+  if (!plan || n_env * d <= 0 || max_iter < 1 || check_every < 1) { cc_set_last_error("lanczos: bad arguments"); return CC_ERR_ARG; }
+  if (cc_internal_stage3_n(plan) != n_env * d) { cc_set_last_error("lanczos: operator size does not match the whitening"); return CC_ERR_DIM; }
+  return lanczos_driver(cc_stream(stream), cc_internal_stage3_apply, plan, n_env, d, L, Lt, W, Wt, x0, tol_residual,
+                        tol_energy, max_iter, check_every, work, x_out, host_result);
+}
+
+extern "C" int cc_lanczos_lp(void* stream, const void* lp_plan, int64_t n_env, int64_t d, const double* L, const double* Lt,
+                             const double* W, const double* Wt, const double* x0, double tol_residual, double tol_energy,
+                             int max_iter, int check_every, double* work, double* x_out, double* host_result) {
+  if (!lp_plan || n_env * d <= 0 || max_iter < 1 || check_every < 1) { cc_set_last_error("lanczos: bad arguments"); return CC_ERR_ARG; }
+  if (cc_internal_lp_n(lp_plan) != n_env * d) { cc_set_last_error("lanczos: operator size does not match the whitening"); return CC_ERR_DIM; }
+  return lanczos_driver(cc_stream(stream), cc_internal_lp_apply, lp_plan, n_env, d, L, Lt, W, Wt, x0, tol_residual,
+                        tol_energy, max_iter, check_every, work, x_out, host_result);
+}
+
+static int lanczos_driver(hipStream_t st, ApplyFn apply, const void* plan, int64_t n_env, int64_t d, const double* L,
+                          const double* Lt, const double* W, const double* Wt, const double* x0, double tol_residual,
+                          double tol_energy, int max_iter, int check_every, double* work, double* x_out,
+                          double* host_result) {
   const int64_t n = n_env * d;
-  if (!plan || n <= 0 || max_iter < 1 || check_every < 1) { cc_set_last_error("lanczos: bad arguments"); return CC_ERR_ARG; }
-  if (cc_internal_stage3_n(plan) != n) { cc_set_last_error("lanczos: operator size does not match the whitening"); return CC_ERR_DIM; }
   Whitening wh{n_env, d, L, Lt, W, Wt};
   double* Q = work;
   double* w = Q + (int64_t)(max_iter + 1) * n;
@@ -262,7 +286,7 @@ extern "C" int cc_lanczos_f64(void* stream, const void* plan, int64_t n_env, int
   int m = 0;
   for (int j = 0; j < max_iter; ++j) {
     double* qj = Q + (int64_t)j * n;
-    rc = op_whitened(st, plan, wh, qj, t1, t2, w);
+    rc = op_whitened(st, apply, plan, wh, qj, t1, t2, w);
     if (rc) return rc;
     rc = cc_dot_f64(st, n, qj, w, alpha + j);
     if (rc) return rc;
@@ -322,4 +346,203 @@ extern "C" int cc_lanczos_f64(void* stream, const void* plan, int64_t n_env, int
   host_result[1] = resid;
   host_result[2] = (double)m;
   return CC_OK;
+}
+
+// ---------------------------------------------------------------- Jacobi-Davidson pieces
+namespace {
+// out[j] = sum_i U[j][i] v[i] for j < k  (one block per basis vector, fixed-order tree)
+__global__ void __launch_bounds__(256) jd_dots_kernel(int64_t n, const double* __restrict__ U, const double* __restrict__ v,
+                                                      double* __restrict__ out, int stride_out) {
+  __shared__ double red[256];
+  const double* u = U + (int64_t)blockIdx.x * n;
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 256) acc += u[i] * v[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[(int64_t)blockIdx.x * stride_out] = red[0];
+}
+
+// y = a*x + b*y with a, b read from device scalars (a = sa[0]*fa, b = sb ? sb[0]*fb : fb)
+__global__ void cg_axpby_kernel(int64_t n, const double* __restrict__ sa, double fa, const double* __restrict__ x,
+                                const double* __restrict__ sb, double fb, double* __restrict__ y) {
+  const double a = (sa ? sa[0] : 1.0) * fa, b = (sb ? sb[0] : 1.0) * fb;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = a * x[i] + (b == 0.0 ? 0.0 : b * y[i]);
+}
+
+// ratio[0] = num[0] / den[0] (0 when den == 0)
+__global__ void ratio_kernel(const double* num, const double* den, double* out) {
+  out[0] = den[0] != 0.0 ? num[0] / den[0] : 0.0;
+}
+
+// lowest eigenpair of the k x k pencil (Hs, Ns), k <= 32, single thread: Cholesky of Ns, cyclic Jacobi
+// on L^-1 Hs L^-T, back-transform.  out[0] = value, vec[0..k) N-normalised coefficients.
+__global__ void small_geneig_kernel(int k, const double* __restrict__ Hs, const double* __restrict__ Ns,
+                                    double* __restrict__ out, double* __restrict__ vec) {
+  if (threadIdx.x != 0) return;
+  double Lm[32][32], C[32][32], V[32][32], Li[32][32];
+  for (int i = 0; i < k; ++i)
+    for (int j = 0; j < k; ++j) { Lm[i][j] = 0.0; Li[i][j] = 0.0; V[i][j] = (i == j) ? 1.0 : 0.0; }
+  for (int j = 0; j < k; ++j) {
+    double s = Ns[j * k + j];
+    for (int c = 0; c < j; ++c) s -= Lm[j][c] * Lm[j][c];
+    Lm[j][j] = sqrt(fmax(s, 1e-300));
+    for (int i = j + 1; i < k; ++i) {
+      double t = Ns[i * k + j];
+      for (int c = 0; c < j; ++c) t -= Lm[i][c] * Lm[j][c];
+      Lm[i][j] = t / Lm[j][j];
+    }
+  }
+  for (int c = 0; c < k; ++c)
+    for (int i = c; i < k; ++i) {
+      double v = (i == c) ? 1.0 : 0.0;
+      for (int t = c; t < i; ++t) v -= Lm[i][t] * Li[t][c];
+      Li[i][c] = v / Lm[i][i];
+    }
+  for (int i = 0; i < k; ++i)
+    for (int j = 0; j < k; ++j) {
+      double acc = 0.0;
+      for (int a = 0; a <= i; ++a)
+        for (int b = 0; b <= j; ++b) acc += Li[i][a] * Hs[a * k + b] * Li[j][b];
+      C[i][j] = acc;
+    }
+  for (int sweep = 0; sweep < 50; ++sweep) {
+    double off = 0.0;
+    for (int i = 0; i < k; ++i)
+      for (int j = i + 1; j < k; ++j) off += C[i][j] * C[i][j];
+    if (off < 1e-30) break;
+    for (int p = 0; p < k; ++p)
+      for (int q = p + 1; q < k; ++q) {
+        if (C[p][q] == 0.0) continue;
+        double th = (C[q][q] - C[p][p]) / (2.0 * C[p][q]);
+        double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+        double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        for (int r = 0; r < k; ++r) {
+          double a = C[r][p], b = C[r][q];
+          C[r][p] = c * a - s * b;
+          C[r][q] = s * a + c * b;
+        }
+        for (int r = 0; r < k; ++r) {
+          double a = C[p][r], b = C[q][r];
+          C[p][r] = c * a - s * b;
+          C[q][r] = s * a + c * b;
+        }
+        for (int r = 0; r < k; ++r) {
+          double a = V[r][p], b = V[r][q];
+          V[r][p] = c * a - s * b;
+          V[r][q] = s * a + c * b;
+        }
+      }
+  }
+  int best = 0;
+  for (int i = 1; i < k; ++i)
+    if (C[i][i] < C[best][best]) best = i;
+  out[0] = C[best][best];
+  for (int i = 0; i < k; ++i) {  // coefficients = L^-T v
+    double acc = 0.0;
+    for (int j = i; j < k; ++j) acc += Li[j][i] * V[j][best];
+    vec[i] = acc;
+  }
+}
+}  // namespace
+
+extern "C" int cc_basis_dots_f64(void* stream, int k, int64_t n, const double* U, const double* v, double* out, int stride_out) {
+  if (k <= 0) return CC_OK;
+  jd_dots_kernel<<<k, 256, 0, cc_stream(stream)>>>(n, U, v, out, stride_out);
+  CC_LAUNCH_CHECK();
+  return CC_OK;
+}
+
+extern "C" int cc_basis_combine_f64(void* stream, int k, int64_t n, const double* U, const double* c, double* out) {
+  lz_combine_kernel<<<(unsigned)((n + 15) / 16), 256, 0, cc_stream(stream)>>>(k, n, U, c, out, 1);
+  CC_LAUNCH_CHECK();
+  return CC_OK;
+}
+
+extern "C" int cc_small_geneig_f64(void* stream, int k, const double* Hs, const double* Ns, double* out, double* vec) {
+  if (k < 1 || k > 32) { cc_set_last_error("small_geneig: 1 <= k <= 32"); return CC_ERR_ARG; }
+  small_geneig_kernel<<<1, 64, 0, cc_stream(stream)>>>(k, Hs, Ns, out, vec);
+  CC_LAUNCH_CHECK();
+  return CC_OK;
+}
+
+extern "C" int cc_jd_work_size(int64_t n, int64_t* out) {
+  *out = 10 * n + 16;
+  return CC_OK;
+}
+
+// Approximate solution of the projected correction equation
+//   (I - u u^T)(W Hlp W^T - rho)(I - u u^T) y = -(I - u u^T) W r,  u = L^T x / |L^T x|,  t = W^T (I - u u^T) y
+// by `iters` CG steps on the low-precision operator (all scalars on the device).
+extern "C" int cc_jd_correction_lp(void* stream, const void* lp_plan, int64_t n_env, int64_t d, const double* Lt,
+                                   const double* W, const double* Wt, double rho, const double* x, const double* r,
+                                   int iters, double* work, double* t_out) {
+  hipStream_t st = cc_stream(stream);
+  const int64_t n = n_env * d;
+  if (cc_internal_lp_n(lp_plan) != n) { cc_set_last_error("jd: operator size mismatch"); return CC_ERR_DIM; }
+  double* u = work;
+  double* y = u + n;
+  double* res = y + n;
+  double* p = res + n;
+  double* ap = p + n;
+  double* t1 = ap + n;
+  double* t2 = t1 + n;
+  double* tmp = t2 + n;
+  double* sc = tmp + 2 * n;  // scalars: 0 uu, 1 rr, 2 rr_new, 3 pAp, 4 alpha, 5 beta, 6 dot
+  const unsigned gb = cc_blocks(n, 256);
+  int rc;
+  Whitening wh{n_env, d, nullptr, Lt, W, Wt};
+  // u = normalized L^T x
+  rc = cc_skinny_f64(st, n_env, 2, Lt, (int)d, x, tmp);
+  if (rc) return rc;
+  lz_normalize_kernel<<<1, 1024, 0, st>>>(n, tmp, u, nullptr);
+  CC_LAUNCH_CHECK();
+  auto project = [&](double* v) -> int {  // v -= u (u . v)
+    rc = cc_dot_f64(st, n, u, v, sc + 6);
+    if (rc) return rc;
+    cg_axpby_kernel<<<gb, 256, 0, st>>>(n, sc + 6, -1.0, u, nullptr, 1.0, v);
+    CC_LAUNCH_CHECK();
+    return CC_OK;
+  };
+  // res = -P W r ; p = res ; y = 0
+  rc = cc_skinny_f64(st, n_env, 1, W, (int)d, r, res);
+  if (rc) return rc;
+  cg_axpby_kernel<<<gb, 256, 0, st>>>(n, nullptr, -1.0, res, nullptr, 0.0, res);
+  CC_LAUNCH_CHECK();
+  if ((rc = project(res))) return rc;
+  rc = cc_fill_f64(st, n, 0.0, y);
+  if (rc) return rc;
+  cg_axpby_kernel<<<gb, 256, 0, st>>>(n, nullptr, 1.0, res, nullptr, 0.0, p);
+  CC_LAUNCH_CHECK();
+  rc = cc_dot_f64(st, n, res, res, sc + 1);
+  if (rc) return rc;
+  for (int k = 0; k < iters; ++k) {
+    // ap = P (W H W^T - rho) p   (p is kept in the projected space)
+    rc = op_whitened(st, cc_internal_lp_apply, lp_plan, wh, p, t1, t2, ap);
+    if (rc) return rc;
+    cg_axpby_kernel<<<gb, 256, 0, st>>>(n, nullptr, -rho, p, nullptr, 1.0, ap);
+    CC_LAUNCH_CHECK();
+    if ((rc = project(ap))) return rc;
+    rc = cc_dot_f64(st, n, p, ap, sc + 3);
+    if (rc) return rc;
+    ratio_kernel<<<1, 1, 0, st>>>(sc + 1, sc + 3, sc + 4);                 // alpha = rr / pAp
+    CC_LAUNCH_CHECK();
+    cg_axpby_kernel<<<gb, 256, 0, st>>>(n, sc + 4, 1.0, p, nullptr, 1.0, y);  // y += alpha p
+    CC_LAUNCH_CHECK();
+    cg_axpby_kernel<<<gb, 256, 0, st>>>(n, sc + 4, -1.0, ap, nullptr, 1.0, res);  // res -= alpha ap
+    CC_LAUNCH_CHECK();
+    rc = cc_dot_f64(st, n, res, res, sc + 2);
+    if (rc) return rc;
+    ratio_kernel<<<1, 1, 0, st>>>(sc + 2, sc + 1, sc + 5);                 // beta = rr_new / rr
+    CC_LAUNCH_CHECK();
+    cg_axpby_kernel<<<gb, 256, 0, st>>>(n, nullptr, 1.0, res, sc + 5, 1.0, p);   // p = res + beta p
+    CC_LAUNCH_CHECK();
+    CC_HIP(hipMemcpyAsync(sc + 1, sc + 2, sizeof(double), hipMemcpyDeviceToDevice, st));
+  }
+  if ((rc = project(y))) return rc;
+  return cc_skinny_f64(st, n_env, 2, Wt, (int)d, y, t_out);                // t = W^T P y
 }

@@ -1,0 +1,151 @@
+"""The precision form of the solve: FP16 / BF16 stage-3 matvecs with FP32 accumulation inside the
+eigensolver, FP64 for everything that fixes the answer.
+
+LowPrecisionOperator -- the stage-3 operator of one solve with every distinct slab channel
+    scaled by 1/max|slab| and rounded once to FP16 or BF16 (stored unpadded); the scales fold
+    into each term's d x d site-operator coefficients.
+MixedSolver -- (1) Lanczos on the whitened low-precision operator (the FP64 whitening of the
+    exact N) until the Ritz value stagnates; (2) Jacobi-Davidson refinement in FP64: each outer
+    step applies the FP64 H and N to one new basis vector, solves the small generalized
+    eigenproblem of the basis on the device (FP64 Rayleigh-Ritz), and computes the correction
+    from a fixed number of CG steps on the projected, shifted low-precision operator.  The
+    energy reported is the FP64 Rayleigh quotient of the final Ritz vector.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .device import empty, ptr, stream
+from .solver import NormalizationFactor
+
+DTYPES = {"f16": _lib.CC_F16, "bf16": _lib.CC_BF16}
+
+
+class LowPrecisionOperator:
+    def __init__(self, operator, dtype="bf16", groups=None, split=True):
+        if not operator.use_fused:
+            raise NotImplementedError("the low-precision kernel needs P = Q in {4, 9, 16, 25, 36} and d = 2")
+        self.base, self.dtype = operator, dtype
+        self.n, self.n_alpha, self.P, self.d = operator.n, operator.n_alpha, operator.P, operator.d
+        code = DTYPES[dtype]
+        a_ids, b_ids, a_list, b_list, rows = {}, {}, [], [], []
+        for k, (A, B, O) in enumerate(zip(operator._A, operator._B, operator._O)):
+            if id(A) not in a_ids:
+                a_ids[id(A)] = len(a_list)
+                a_list.append(A)
+            if id(B) not in b_ids:
+                b_ids[id(B)] = len(b_list)
+                b_list.append(B)
+            rows.append((a_ids[id(A)], k, b_ids[id(B)], O))
+        rows.sort(key=lambda r: (r[0], r[1]))
+        self._slabs, scales = [], []
+        sc = empty((len(a_list) + len(b_list), 513))
+        for i, ch in enumerate(a_list + b_list):
+            buf = torch.empty(ch.size(), dtype=torch.float16 if dtype == "f16" else torch.bfloat16, device=sc.device)
+            _lib.call("cc_stage3_lp_convert", stream(), code, ch.size(), ptr(ch._t), ptr(sc[i]), ptr(buf))
+            self._slabs.append(buf)
+        scales = sc[:, 0].tolist()
+        sa, sb = scales[:len(a_list)], scales[len(a_list):]
+        coef = []
+        for (a, _, b, O) in rows:
+            m = np.identity(self.d) if O is None else O.toArray()
+            coef.extend((sa[a] * sb[b] * m).ravel().tolist())
+        self.groups = int(groups if groups is not None else max(1, min((self.n_alpha + 15) // 16, 256)))
+        self._work = torch.empty(_lib.query("cc_stage3_lp_work_size", self.n_alpha, self.P, self.d, self.groups),
+                                 dtype=torch.float32, device=sc.device)
+        self._ptrs_a = _lib.ptr_array([t.data_ptr() for t in self._slabs[:len(a_list)]])
+        self._ptrs_b = _lib.ptr_array([t.data_ptr() for t in self._slabs[len(a_list):]])
+        h = ctypes.c_void_p()
+        _lib.call("cc_stage3_lp_plan_create", ctypes.byref(h), code, len(a_list), self._ptrs_a, len(b_list), self._ptrs_b,
+                  len(rows), _lib.i32_array([r[0] for r in rows]), _lib.i32_array([r[2] for r in rows]),
+                  _lib.f32_array(coef), self.n_alpha, self.P, self.d, self.groups, 1 if split else 0, ptr(self._work))
+        self.plan = h
+        self.bytes_per_matvec = sum(t.numel() * t.element_size() for t in self._slabs) + 2 * self.n * 8
+
+    def apply_flat(self, x, y):
+        _lib.call("cc_stage3_lp_plan_apply", stream(), self.plan, ptr(x), ptr(y))
+
+    def __del__(self):
+        if getattr(self, "plan", None) is not None:
+            try:
+                _lib.load().cc_stage3_lp_plan_destroy(self.plan)
+            except Exception:
+                pass
+
+
+class MixedSolver:
+    """Lowest eigenpair of (H, N) to FP64 accuracy with low-precision matvecs doing the bulk."""
+
+    def __init__(self, h_operator, n_operator, dtype="bf16", factor=None, groups=None, split=True):
+        self.H, self.N = h_operator, n_operator
+        self.Hlp = LowPrecisionOperator(h_operator, dtype, groups=groups, split=split)
+        self.factor = factor if factor is not None else NormalizationFactor(n_operator)
+        self.n = h_operator.n
+        h_operator._ensure_fused()
+        n_operator._ensure_fused()
+        self._hplan, self._nplan = h_operator.plan_handle(), n_operator.plan_handle()
+        self._lz_work, self._lz_max = None, 0
+        self._jd_work = empty((_lib.query("cc_jd_work_size", self.n),))
+
+    def solve(self, x0, max_lowprec_iter=300, check_every=4, inner_iters=20, max_outer=8, tol_energy=1e-12):
+        from .data import HipData
+        if isinstance(x0, HipData):
+            x0 = x0._t.reshape(-1)
+        f, n = self.factor, self.n
+        max_lowprec_iter = int(min(max_lowprec_iter, n))
+        if self._lz_work is None or self._lz_max < max_lowprec_iter:
+            self._lz_work = empty((_lib.query("cc_lanczos_work_size", n, max_lowprec_iter),))
+            self._lz_max = max_lowprec_iter
+        x = empty((n,))
+        res = _lib.f64_array([0, 0, 0])
+        # (1) low-precision Lanczos until the Ritz value stops moving at the precision floor
+        _lib.call("cc_lanczos_lp", stream(), self.Hlp.plan, f.n_env, f.d, ptr(f.L), ptr(f.Lt), ptr(f.W), ptr(f.Wt),
+                  ptr(x0), 0.0, 1e-9, max_lowprec_iter, int(check_every), ptr(self._lz_work), ptr(x), res)
+        lp_iters = int(res[2])
+        # (2) Jacobi-Davidson in FP64
+        kmax = max_outer + 1
+        U, HU, NU = empty((kmax, n)), empty((kmax, n)), empty((kmax, n))
+        small = empty((kmax + 1,))
+        xr, r, t = empty((n,)), empty((n,)), empty((n,))
+        U[0].copy_(x)
+        k, rho, prev = 0, None, None
+        fp64_passes = 0
+        for outer in range(max_outer):
+            # FP64 H and N on the newest basis vector, then the projected pencil
+            _lib.call("cc_stage3_plan_apply", stream(), self._hplan, ptr(U[k]), ptr(HU[k]))
+            _lib.call("cc_stage3_plan_apply", stream(), self._nplan, ptr(U[k]), ptr(NU[k]))
+            fp64_passes += 1
+            k += 1
+            hs, ns = empty((k, k)), empty((k, k))
+            for j in range(k):
+                _lib.call("cc_basis_dots_f64", stream(), k, n, ptr(U), ptr(HU[j]), ptr(hs[:, j]), k)
+                _lib.call("cc_basis_dots_f64", stream(), k, n, ptr(U), ptr(NU[j]), ptr(ns[:, j]), k)
+            _lib.call("cc_small_geneig_f64", stream(), k, ptr(hs), ptr(ns), ptr(small[0:1]), ptr(small[1:1 + k]))
+            rho = float(small[0].item())
+            coeffs = small[1:1 + k].contiguous()
+            _lib.call("cc_basis_combine_f64", stream(), k, n, ptr(U), ptr(coeffs), ptr(xr))
+            if prev is not None and abs(prev - rho) <= tol_energy * abs(rho):
+                break
+            prev = rho
+            if outer == max_outer - 1:
+                break
+            # residual r = H xr - rho N xr from the basis products
+            _lib.call("cc_basis_combine_f64", stream(), k, n, ptr(HU), ptr(coeffs), ptr(r))
+            _lib.call("cc_basis_combine_f64", stream(), k, n, ptr(NU), ptr(coeffs), ptr(t))
+            _lib.call("cc_axpby_f64", stream(), n, -rho, ptr(t), 1.0, ptr(r))
+            _lib.call("cc_jd_correction_lp", stream(), self.Hlp.plan, f.n_env, f.d, ptr(f.Lt), ptr(f.W), ptr(f.Wt),
+                      float(rho), ptr(xr), ptr(r), int(inner_iters), ptr(self._jd_work), ptr(t))
+            # orthogonalise the correction against the basis (twice, Euclidean) and normalise
+            for _ in range(2):
+                c = empty((k,))
+                _lib.call("cc_basis_dots_f64", stream(), k, n, ptr(U), ptr(t), ptr(c), 1)
+                tmp = empty((n,))
+                _lib.call("cc_basis_combine_f64", stream(), k, n, ptr(U), ptr(c), ptr(tmp))
+                _lib.call("cc_axpby_f64", stream(), n, -1.0, ptr(tmp), 1.0, ptr(t))
+            nrm = empty((1,))
+            _lib.call("cc_dot_f64", stream(), n, ptr(t), ptr(t), ptr(nrm))
+            _lib.call("cc_axpby_f64", stream(), n, 1.0 / np.sqrt(float(nrm.item())), ptr(t), 0.0, ptr(U[k]))
+        return xr, rho, dict(lowprec_iterations=lp_iters, outer_steps=k, fp64_passes=fp64_passes,
+                             lowprec_ritz=float(res[0]))

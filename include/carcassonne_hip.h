@@ -167,6 +167,39 @@ int cc_lanczos_f64(void* stream, const void* plan, int64_t n_env, int64_t d, con
                    double tol_residual, double tol_energy, int max_iter, int check_every,
                    double* work, double* x_out, double* host_result);
 
+/* ---- precision form: FP16 / BF16 stage-3 matvec with FP32 accumulation ---------------
+ * (no reference counterpart; the same arithmetic as formDenseStage3 at reduced precision)
+ * cc_stage3_lp_convert: out = slab / max|slab| rounded to dtype; scale_out[0] = max|slab|.
+ * cc_stage3_lp_plan_create: slabs stored unpadded (n_alpha, P, P); term_coef[t] is the d x d
+ * matrix scale_a * scale_b * O_t (identity for a normalization-type term); split != 0 carries the
+ * vector and the intermediate as hi + lo pairs (only the slabs are rounded); `work` holds
+ * cc_stage3_lp_work_size floats.  Apply computes y (FP64) = H_lp x (FP64). */
+int cc_stage3_lp_convert(void* stream, int dtype, int64_t n, const double* slab, double* scale_out, void* out);
+int cc_stage3_lp_work_size(int64_t n_alpha, int64_t P, int64_t d, int groups, int64_t* out);
+int cc_stage3_lp_plan_create(void** out, int dtype, int n_a, const void* const* A, int n_b, const void* const* B,
+                             int n_terms, const int* term_a, const int* term_b, const float* term_coef,
+                             int64_t n_alpha, int64_t P, int64_t d, int groups, int split, float* work);
+int cc_stage3_lp_plan_apply(void* stream, const void* plan, const double* x, double* y);
+int cc_stage3_lp_plan_destroy(void* plan);
+
+/* Lanczos (as cc_lanczos_f64) on a low-precision plan. */
+int cc_lanczos_lp(void* stream, const void* lp_plan, int64_t n_env, int64_t d, const double* L, const double* Lt,
+                  const double* W, const double* Wt, const double* x0, double tol_residual, double tol_energy,
+                  int max_iter, int check_every, double* work, double* x_out, double* host_result);
+
+/* Jacobi-Davidson refinement pieces (FP64):
+ * out[j*stride_out] = U[j] . v for j < k;  out = sum_j c[j] U[j];  lowest eigenpair of the k x k
+ * pencil (Hs, Ns), k <= 32, out[0] = value, vec = N-normalised coefficients;  the correction
+ * t = W^T P y with y from `iters` CG steps on P (W H_lp W^T - rho) P y = -P W r (P = I - u u^T,
+ * u = L^T x / |L^T x|); `work` holds cc_jd_work_size doubles. */
+int cc_basis_dots_f64(void* stream, int k, int64_t n, const double* U, const double* v, double* out, int stride_out);
+int cc_basis_combine_f64(void* stream, int k, int64_t n, const double* U, const double* c, double* out);
+int cc_small_geneig_f64(void* stream, int k, const double* Hs, const double* Ns, double* out, double* vec);
+int cc_jd_work_size(int64_t n, int64_t* out);
+int cc_jd_correction_lp(void* stream, const void* lp_plan, int64_t n_env, int64_t d, const double* Lt,
+                        const double* W, const double* Wt, double rho, const double* x, const double* r,
+                        int iters, double* work, double* t_out);
+
 #ifdef __cplusplus
 }
 #endif
