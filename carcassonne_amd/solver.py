@@ -113,15 +113,25 @@ def _axpby(a, x, b, y):
 
 
 class _DenseInverse:
-    """x -> N^-1 x through W = L^-1 of the formed matrix: N^-1 = W^T W."""
+    """x -> N^-1 x through W = L^-1 of the formed matrix: N^-1 = W^T W.  A normalization matrix
+    that is only numerically semi-definite (an environment right after a bandwidth increase) is
+    factored with escalating relative diagonal shifts 1e-14, 1e-12, 1e-10 (the reference's LU
+    inverts it unshifted)."""
 
     def __init__(self, matrix):
         n = matrix.shape[0]
-        L = matrix.tensor().clone()
-        info = torch.zeros(1, dtype=torch.int32, device=L.device)
+        base = matrix.tensor()
+        trace = float(torch.diagonal(base).sum().item())
         work = empty((_lib.query("cc_linalg_work_size", n),))
-        _lib.call("cc_cholesky_f64", stream(), n, ptr(L), ptr(info), ptr(work))
-        if int(info.item()) != 0:
+        info = torch.zeros(1, dtype=torch.int32, device=base.device)
+        for shift in (0.0, 1e-14, 1e-12, 1e-10):
+            L = base.clone()
+            if shift:
+                L.view(-1)[:: n + 1] += shift * abs(trace) / n
+            _lib.call("cc_cholesky_f64", stream(), n, ptr(L), ptr(info), ptr(work))
+            if int(info.item()) == 0:
+                break
+        else:
             raise ValueError("the normalization matrix is not positive definite (pivot {}); the HIP path "
                              "factors N by Cholesky".format(int(info.item())))
         self.W = empty(L.shape)

@@ -1,0 +1,1 @@
+from ._2d import System  # noqa: F401

@@ -1,0 +1,202 @@
+"""GPU parity of the System layer (absorption, observables, solves, compression, gauge moves,
+driver) against the oracle on identical seeded tensors, and the reference's known answers
+(carcassonne/tests/test_system.py, test_simulator_2d_in_1d.py) computed on the GPU."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(seed, maximum_dimension=2, O=None):
+    from oracle.system import System as OSystem
+    from tests.oracle_bridge import system_to_product
+    osys = OSystem.newRandom(np.random.default_rng(seed), maximum_dimension=maximum_dimension, O=O)
+    return system_to_product(osys), osys
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_walk_absorption_matches_oracle(gpu, seed):
+    """contractTowards (normalizeAxis + three absorptions) on a random system; the polar factor
+    is unique for full-rank matricizations, so raw tensors must agree, not only energies."""
+    from tests.oracle_bridge import compare_systems
+    psys, osys = _pair(seed, maximum_dimension=3)
+    rng = np.random.default_rng(100 + seed)
+    for direction in rng.integers(0, 4, 4):
+        psys.contractTowards(int(direction))
+        osys.contractTowards(int(direction))
+        assert compare_systems(psys, osys) < 1e-8
+    e_p, n_p = psys.computeExpectationAndNormalization()
+    e_o, n_o = osys.computeExpectationAndNormalization()
+    assert abs(e_p - e_o) <= 1e-9 * abs(e_o) and abs(n_p - n_o) <= 1e-9 * abs(n_o)
+
+
+def test_observables_match_oracle(gpu):
+    psys, osys = _pair(11, maximum_dimension=2)
+    for d in (0, 1, 2, 3):
+        psys.contractTowards(d)
+        osys.contractTowards(d)
+    assert abs(psys.computeCenterSiteExpectation() - osys.computeCenterSiteExpectation()) < 1e-9
+    assert abs(psys.computeOneSiteExpectation() - osys.computeOneSiteExpectation()) < 1e-9
+    assert abs(psys.computeEstimatedOneSiteExpectation(0) - osys.computeEstimatedOneSiteExpectation(0)) < 1e-9
+
+
+def test_minimize_expectation_relaxover_matches_oracle(gpu):
+    """The reference's solve (relaxOver, k=3, tol 1e-7, 100 multiplications) on a grown ferromagnet
+    environment: same number of restarts and the same energy."""
+    from carcassonne_amd.data import HipData
+    from carcassonne_amd.system import System
+    from oracle.system import System as OSystem
+    Z = np.diag([1.0, -1.0])
+    X = np.array([[0.0, 1.0], [1.0, 0.0]])
+    osys = OSystem.newTrivialWithSimpleSparseOperator(O=-0.5 * X, OO_LR=[Z, -Z], OO_UD=[Z, -Z])
+    psys = System.newTrivialWithSimpleSparseOperator(O=HipData.fromArray(-0.5 * X),
+                                                     OO_LR=[HipData.Z, -HipData.Z], OO_UD=[HipData.Z, -HipData.Z])
+    for sys_ in (psys, osys):
+        for d in (0, 2, 1, 3):
+            sys_.increaseBandwidth(d % 2, by=1, do_as_much_as_possible=True) if False else None
+    for d in (0, 1, 2, 3, 0):
+        osys.contractTowards(d)
+        psys.contractTowards(d)
+    osys.minimizeExpectation()
+    psys.minimizeExpectation()
+    assert psys.last_solve["restarts"] == osys.last_solve["restarts"]
+    e_p, e_o = psys.computeExpectation(), osys.computeExpectation()
+    assert abs(e_p - e_o) <= 1e-9 * abs(e_o)
+
+
+def test_gauge_moves_leave_expectations_invariant(gpu):
+    """normalizeCorner/Side/Center... (reference test_system.py:276-305)."""
+    psys, _ = _pair(5, maximum_dimension=3)
+    psys.contractTowards(0)
+    psys.contractTowards(1)
+    e1, n1 = psys.computeExpectationAndNormalization()
+    psys.normalizeCornerAndDenormalizeSide(2, 1)
+    psys.normalizeSideAndDenormalizeCenter(3)
+    psys.normalizeCenterAndDenormalizeSide(0)
+    e2, n2 = psys.computeExpectationAndNormalization()
+    assert abs(e2 / e1 - 1) < 1e-9 and abs(n2 / n1 - 1) < 1e-9
+
+
+def test_compression_exact_at_full_rank(gpu):
+    """compressCornerStateTowards to the current dimension is exact (reference test_system.py:64-81)."""
+    from carcassonne_amd.sparse import Identity
+    psys, _ = _pair(8, maximum_dimension=2)
+    psys.contractTowards(0)
+    psys.contractTowards(1)
+    e1, n1 = psys.computeExpectationAndNormalization()
+    for corner_id in range(4):
+        for direction in range(2):
+            dim = psys.corners[corner_id][Identity()].shape[3 * direction]
+            psys.compressCornerStateTowards(corner_id, direction, dim)
+    e2, n2 = psys.computeExpectationAndNormalization()
+    assert abs(e2 / e1 - 1) < 1e-7 and abs(n2 / n1 - 1) < 1e-7
+
+
+def test_compression_recovers_enlarged_bond(gpu):
+    """Enlarge a corner-side bond by an isometry, then compress back (reference test_system.py:40-63)."""
+    from carcassonne_amd.data import HipData
+    from carcassonne_amd.sparse import Identity, mapOverSparseData
+    from carcassonne_amd.system._2d import sideFromCorner
+    psys, _ = _pair(9, maximum_dimension=2)
+    for i in range(4):
+        psys.contractTowards(i)
+    e1, n1 = psys.computeExpectationAndNormalization()
+    corner_id, direction = 1, 0
+    axis = 3 * direction
+    old = psys.corners[corner_id][Identity()].shape[axis]
+    new = 2 * (old + 1)
+    enl, enl_c = HipData.newEnlargener(old, new)
+    psys.corners[corner_id] = mapOverSparseData(lambda d: d.absorbMatrixAt(axis, enl).absorbMatrixAt(axis + 1, enl_c),
+                                                psys.corners[corner_id])
+    side_id = sideFromCorner(corner_id, direction)
+    saxis = 3 - axis
+    psys.sides[side_id] = mapOverSparseData(lambda d: d.absorbMatrixAt(saxis, enl_c).absorbMatrixAt(saxis + 1, enl),
+                                            psys.sides[side_id])
+    e2, n2 = psys.computeExpectationAndNormalization()
+    assert abs(n2 / n1 - 1) < 1e-9 and abs(e2 / e1 - 1) < 1e-9
+    psys.compressCornerStateTowards(corner_id, direction, old)
+    e3, n3 = psys.computeExpectationAndNormalization()
+    assert abs(n3 / n1 - 1) < 1e-6 and abs(e3 / e1 - 1) < 1e-6
+
+
+@pytest.mark.parametrize("walk", [[0], [1, 2], [0, 0, 3], [2, 3, 1, 0, 1]])
+def test_identity_operator_walk_is_width_times_height(gpu, walk):
+    """Silly field walk (reference test_system.py:308-358): E = width*height exactly, N = 1."""
+    from carcassonne_amd.data import HipData
+    from carcassonne_amd.system import System
+    system = System.newTrivialWithSimpleSparseOperator(O=HipData.newIdentity(1))
+    width = height = 1
+    for d in walk:
+        system.contractTowards(d)
+        if d in (0, 2):
+            width += 1
+        else:
+            height += 1
+    assert system.computeExpectation() == pytest.approx(width * height, abs=1e-12)
+    assert system.computeNormalization() == pytest.approx(1.0, abs=1e-12)
+
+
+def test_magnetic_field_product_state_walk(gpu):
+    """Z field with up/down product states (reference test_system.py:360-439)."""
+    from carcassonne_amd.data import HipData
+    from carcassonne_amd.system import System
+    system = System.newTrivialWithSimpleSparseOperator(O=HipData.Z)
+    up = HipData.fromArray(np.array([[[[[1.0, 0.0]]]]]))
+    down = HipData.fromArray(np.array([[[[[0.0, 1.0]]]]]))
+    states, spins = (up, down), (1, -1)
+    rng = np.random.default_rng(3)
+    UL = U = UR = L_ = R_ = DL = D = DR = 0
+    for _ in range(6):
+        direction, spin = int(rng.integers(0, 4)), int(rng.integers(0, 2))
+        system.setStateCenter(states[spin])
+        system.contractTowards(direction)
+        if direction == 0:
+            R_ += spins[spin]; UR += U; DR += D
+        elif direction == 1:
+            U += spins[spin]; UL += L_; UR += R_
+        elif direction == 2:
+            L_ += spins[spin]; UL += U; DL += D
+        else:
+            D += spins[spin]; DL += L_; DR += R_
+    final = 1
+    system.setStateCenter(states[final])
+    total = UL + U + UR + L_ + R_ + spins[final] + DL + D + DR
+    assert system.computeExpectation() == pytest.approx(total, abs=1e-10)
+    assert system.computeNormalization() == pytest.approx(1.0, abs=1e-12)
+
+
+@pytest.mark.parametrize("direction", [0, 1])
+def test_2d_in_1d_transverse_ising_known_answer(gpu, direction):
+    """-1.0000250001562545 to 7 places (reference test_simulator_2d_in_1d.py:36-47), the whole
+    sweep (bandwidth increases, contractions, relaxOver solves, one-site observables) on the GPU."""
+    from carcassonne_amd.data import HipData
+    from carcassonne_amd.system import System
+    from carcassonne_amd import policies as P
+    X, Z = HipData.X, HipData.Z
+    if direction == 0:
+        system = System.newTrivialWithSimpleSparseOperator(O=-Z, OO_LR=[X, X * -0.01])
+    else:
+        system = System.newTrivialWithSimpleSparseOperator(O=-Z, OO_UD=[X, X * -0.01])
+    system.setPolicy("sweep convergence", P.RelativeStateDifferenceThresholdConvergencePolicy(1e-5))
+    system.setPolicy("run convergence", P.RelativeOneSiteExpectationDifferenceThresholdConvergencePolicy(1e-7))
+    system.setPolicy("bandwidth increase", P.OneDirectionIncrementBandwidthIncreasePolicy(direction, 2))
+    system.setPolicy("contraction", P.RepeatPatternContractionPolicy([0 + direction, 2 + direction]))
+    system.runUntilConverged(maximum_number_of_runs=30, maximum_number_of_steps=200)
+    assert round(system.computeOneSiteExpectation() - (-1.0000250001562545), 7) == 0
+
+
+def test_2d_in_1d_ferromagnet_known_answer(gpu):
+    """-1 (reference test_simulator_2d_in_1d.py:23-34)."""
+    from carcassonne_amd.data import HipData
+    from carcassonne_amd.system import System
+    from carcassonne_amd import policies as P
+    Z = HipData.Z
+    system = System.newTrivialWithSimpleSparseOperator(OO_LR=[Z, -Z])
+    system.setPolicy("sweep convergence", P.RelativeOneSiteExpectationDifferenceThresholdConvergencePolicy(1e-7))
+    system.setPolicy("run convergence", P.RelativeOneSiteExpectationDifferenceThresholdConvergencePolicy(1e-7))
+    system.setPolicy("bandwidth increase", P.OneDirectionIncrementBandwidthIncreasePolicy(0))
+    system.setPolicy("contraction", P.RepeatPatternContractionPolicy([0, 2]))
+    system.runUntilConverged(maximum_number_of_runs=20, maximum_number_of_steps=100)
+    assert round(system.computeOneSiteExpectation() - (-1.0), 7) == 0
