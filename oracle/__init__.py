@@ -21,8 +21,10 @@ Modules and the reference code each one follows:
                    Lanczos / generalized-eigh reference for the GPU solvers
 * ``system``    -- the 2D System, its observables, sweep driver and policies
                    (carcassonne/system/_2d.py, system/base.py, policies.py)
-* ``envgen``    -- seeded synthetic environments for the benchmark shapes
+* ``policies``  -- the sweep policies (carcassonne/policies.py:46-242)
 
 Pinning (which check fixes each function) is recorded in each module's
-docstring; functions without an independent check say "parity unpinned".
+docstring and exercised by tests/test_oracle_dense.py, test_oracle_linalg.py and
+test_oracle_system.py; functions without an independent check say "parity
+unpinned".
 """

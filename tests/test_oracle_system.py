@@ -1,6 +1,7 @@
 """CPU pins of the oracle: the reference's own known answers (its tests hold no fixtures),
 closed forms, and mathematical identities.  A plausible mistake in a tag rule, an index map, a
 sign or the eigensolver changes at least one of these."""
+import copy
 import json
 import math
 import os
@@ -13,7 +14,7 @@ from scipy.special import ellipe
 from oracle import dense, linalg, solver, sparse2d
 from oracle import policies as P
 from oracle.system import System, side_from_corner
-from oracle.tags import Identity
+from oracle.tags import Identity, L, R
 
 X = np.array([[0.0, 1.0], [1.0, 0.0]])
 Z = np.diag([1.0, -1.0])
@@ -112,18 +113,67 @@ def test_matrix_free_equals_dense_and_eigh_pins_relax_over():
     assert abs(th - w) < 1e-11 * abs(w)
 
 
-def test_normalize_axis_identities():
-    """Isometry and inverses (reference test_data.py:98-133)."""
-    rng = np.random.default_rng(2)
-    a = rng.standard_normal((3, 4, 2, 5))
-    for axis in range(4):
-        if np.prod(a.shape) // a.shape[axis] < a.shape[axis]:
-            continue
-        iso, nrm, den = linalg.normalize_axis(a, axis)
-        m = linalg.join(iso, [i for i in range(4) if i != axis], axis)
-        np.testing.assert_allclose(m.T @ m, np.eye(a.shape[axis]), atol=1e-12)
-        np.testing.assert_allclose(linalg.absorb_matrix_at(iso, axis, den), a, atol=1e-12)
-        np.testing.assert_allclose(nrm @ den, np.eye(a.shape[axis]), atol=1e-10)
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_z_field_product_state_walk(seed):
+    """test_system.py:360-439: with O = Z and a Z eigenstate put at the center before every
+    contraction the energy is an exact count of +-1 values.  Contracting towards i moves the
+    center into side i and pushes sides L(i), R(i) into corners i, R(i) (which keep their
+    strips), so the count follows that bookkeeping; H v = E N v holds at the end."""
+    rng = np.random.default_rng(seed)
+    s = System.newTrivialWithSimpleSparseOperator(O=Z)
+    corner, side = [0] * 4, [0] * 4
+    for d in rng.integers(0, 4, size=6):
+        d = int(d)
+        z = 1 if rng.integers(0, 2) else -1
+        s.setStateCenter(np.array([1.0, 0.0] if z > 0 else [0.0, 1.0]).reshape(1, 1, 1, 1, 2))
+        s.contractTowards(d)
+        corner[d] += side[L(d)]
+        corner[R(d)] += side[R(d)]
+        side[d] += z
+    z = 1 if rng.integers(0, 2) else -1
+    v = np.zeros(s.state.shape)
+    v[0, 0, 0, 0, 0 if z > 0 else 1] = 1.0
+    s.setStateCenter(v)
+    total = sum(corner) + sum(side) + z
+    assert abs(s.computeExpectation() - total) < 1e-12
+    op = s.formOperator()
+    np.testing.assert_allclose(op.apply_H(v), total * op.apply_N(v), atol=1e-12)
+
+
+def test_gauge_moves_leave_observables_invariant():
+    """test_system.py:276-305: normalize() and the individual gauge moves change neither the
+    expectation nor the normalization."""
+    rng = np.random.default_rng(3)
+    s = System.newRandom(rng, maximum_dimension=3)
+    for d in (0, 1, 2, 3):
+        s.contractTowards(d)
+    e0, n0 = s.computeExpectationAndNormalization()
+    s.normalize()
+    e1, n1 = s.computeExpectationAndNormalization()
+    assert abs(e1 / e0 - 1) < 1e-10 and abs(n1 / n0 - 1) < 1e-10
+    for d in range(4):
+        s.normalizeCenterAndDenormalizeSide(d)
+    e2, n2 = s.computeExpectationAndNormalization()
+    assert abs(e2 / e0 - 1) < 1e-10 and abs(n2 / n0 - 1) < 1e-10
+
+
+def test_full_eigensolver_minimizes_and_relax_over_agrees():
+    """minimizeExpectationUsingFullEigensolver (_2d.py:498-502) reaches eigh(H, N)[0]; relaxOver
+    from the same state ends between that value and the starting expectation."""
+    from carcassonne_amd import synthetic as syn
+    from tests.oracle_bridge import to_oracle
+    c, sd, st = syn.make_environment(2, 4, model="ising", seed=5)
+    corners, sides, opc = to_oracle(c, sd, syn.operator_center("ising"))
+    s = System(corners, sides, st, opc)
+    op = s.formOperator()
+    w = eigh(op.matrix_H(), op.matrix_N(), eigvals_only=True)[0]
+    e_start = s.computeExpectation()
+    r = copy.copy(s)
+    r.minimizeExpectation(max_mults=400, tol=1e-12)
+    assert w - 1e-9 * abs(w) <= r.computeExpectation() <= e_start + 1e-12
+    assert abs(r.computeExpectation() - w) < 1e-6 * abs(w)
+    s.minimizeExpectationUsingFullEigensolver()
+    assert abs(s.computeExpectation() - w) < 1e-10 * max(1, abs(w))
 
 
 def test_compression_exact_when_rank_suffices():
