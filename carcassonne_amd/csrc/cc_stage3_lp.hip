@@ -1,223 +1,372 @@
 // Low-precision fused stage-3 matvec (FP16 or BF16 operands, FP32 accumulation on matrix cores).
 //
-// Same arithmetic as the FP64 fused kernel.  Slabs are stored unpadded (alpha, P, P), each channel
-// scaled by 1/max|slab| before rounding; the scales fold into the per-term site-operator
-// coefficients.  One workgroup = NT waves (NT = ceil(P/16)); wave n owns the 16-row p' tile n:
-//   stage A  W[idx = s*Q16 + q][p' in tile n] = sum_p V^T[idx][p] A_a[p'][p]   (v_mfma_f32_16x16x16)
-//   stage B  Y[p' in tile n][q'][s'] += sum_q X[p'][q] B_b[q'][q],  X = sum_s O[s'][s] W[s*Q16 + q]
-// The FP32 result of stage A (D[i][j] in lane j + 16(i/4), register i%4) is exactly the stage-B
-// A operand (A[i][k] in lane i + 16(k/4), element k%4) once converted, so W never leaves registers.
-// Per alpha the workgroup stages that alpha's slabs in LDS with coalesced 16-byte loads; MFMA
-// operands are 8-byte LDS reads with zero fill outside the slab.  Partials are FP32 per workgroup
-// and are summed in FP64 in a fixed order.
+//   Y[p'][q'][s'] = sum_t sum_alpha sum_{p,q,s} A_a(t)[alpha][p'][p] B_b(t)[alpha][q'][q] C_t[s'][s] v[p][q][s]
+//
+// Same arithmetic as the FP64 fused kernel.  Each slab channel is scaled by 1/max|slab| and rounded
+// once; the scales fold into the per-term coefficient matrices C_t = scale_a * scale_b * O_t.  Slabs are
+// stored (alpha, P, P) with the per-alpha stride rounded up to 8 elements, so every alpha's slab starts
+// on a 16-byte boundary.
+//
+// Work decomposition.  The terms are cut on the host into chunks whose distinct channels fit one LDS
+// buffer (a chunk never splits the terms of one A channel unless that channel alone overflows).  The grid
+// is (alpha groups) x (chunks); a workgroup has NT = ceil(P/16) waves and walks its contiguous alpha
+// range.  Per alpha it stages the chunk's slabs in LDS with direct global->LDS 16-byte loads, double
+// buffered: the loads of alpha+1 are in flight while the waves compute alpha from the other buffer.
+// Wave np owns the 16-row p' tile np:
+//   stage A  W[idx = s*Q16 + q][p' in tile np] = sum_p V^T[idx][p] A_a[p'][p]     (v_mfma_f32_16x16x32)
+//   stage B  Y[p' in tile np][q'][s'] += sum_q X[p'][q] B_b[q'][q],  X = sum_s C_t[s'][s] W[s*Q16 + q]
+// The FP32 result of stage A (D[i][j] in lane j + 16(i/4), register i%4) is the stage-B A operand
+// (A[i][k] in lane i + 16(k/8), element k%8) once converted, provided K runs in the order of lp_kidx
+// (the B operands are read in the same order), so W never leaves registers.  V^T fragments
+// come pre-scaled from a one-workgroup prepare kernel.  With SPLIT the vector and the stage-B
+// intermediate are carried as hi + lo pairs (two MFMAs each), so only the slabs are rounded.  Each
+// workgroup writes one FP32 partial; a second kernel sums them in FP64 in a fixed order.
 #include "cc_common.h"
 
-#include <type_traits>
+#include <cstring>
+#include <vector>
 
 namespace {
 constexpr int LP_MAX_CH = 16, LP_MAX_TERMS = 48, LP_MAX_D = 4;
-typedef short s16x4 __attribute__((ext_vector_type(4)));
+constexpr int LP_MAX_CHUNKS = 16, LP_CHUNK_SLOTS = 32;
 typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) void lds_void;
 
-struct LPPlan {
-  const void* A[LP_MAX_CH];
-  const void* B[LP_MAX_CH];
-  int n_a, n_b, n_terms;
-  int ta[LP_MAX_TERMS], tb[LP_MAX_TERMS];
-  float coef[LP_MAX_TERMS][LP_MAX_D * LP_MAX_D];  // scale_a * scale_b * O[s'][s]
+// Slab geometry shared by host and device: per-alpha stride (elements), and the number of slab slots
+// one LDS buffer holds: 8 KB per wave of the workgroup (two buffers; 48 KB per workgroup at P = 36).
+__host__ __device__ constexpr int lp_stride(int P) { return (P * P + 7) & ~7; }
+__host__ __device__ constexpr int lp_slots(int P) {
+  return (8192 * ((P + 15) / 16) / (2 * lp_stride(P))) < LP_CHUNK_SLOTS ? (8192 * ((P + 15) / 16) / (2 * lp_stride(P)))
+                                                                       : LP_CHUNK_SLOTS;
+}
+
+// 16-byte pieces past the last slot that unconditional operand reads can touch: rows up to 16*NT - 1 and
+// columns up to 32*KT - 1 of the last slab.
+__host__ __device__ constexpr int lp_pad(int P) {
+  return ((16 * ((P + 15) / 16) - 1) * P + 32 * ((P + 31) / 32) - lp_stride(P)) > 0
+             ? (((16 * ((P + 15) / 16) - 1) * P + 32 * ((P + 31) / 32) - lp_stride(P)) * 2 + 15) / 16 + 1
+             : 1;
+}
+
+struct LPChunk {
+  int n_slots, n_runs;
+  int run_slot[LP_MAX_TERMS];      // LDS slot of run r's A slab
+  int run_t0[LP_MAX_TERMS + 1];    // run r covers chunk-ordered terms [run_t0[r], run_t0[r+1])
+  const void* slot_ptr[LP_CHUNK_SLOTS];
+};
+struct LPDevPlan {
   int64_t n_alpha;
+  int n_chunks, pad;
+  LPChunk chunk[LP_MAX_CHUNKS];
+  int term_slot[LP_MAX_TERMS];     // LDS slot of each (chunk-ordered) term's B slab
+  int term_diag[LP_MAX_TERMS];     // 1 when the term's coefficient matrix is diagonal
+  float coef[LP_MAX_TERMS][LP_MAX_D * LP_MAX_D];
 };
 
 template <typename T> struct LPType;
+typedef __fp16 fp16x2 __attribute__((ext_vector_type(2)));
 template <> struct LPType<_Float16> {
-  typedef f16x4 vec;
+  typedef f16x4 vec4;
+  typedef f16x8 vec8;
   static __device__ inline float round(float x) { return (float)(_Float16)x; }
-  static __device__ inline vec pack(float a, float b, float c, float d) {
-    return (vec){(_Float16)a, (_Float16)b, (_Float16)c, (_Float16)d};
+  static __device__ inline _Float16 cvt(float x) { return (_Float16)x; }
+  // hi + lo of two values: hi truncated to FP16 (one packed conversion), lo = the exact FP32 residual
+  // truncated to FP16; |x - hi - lo| <= 2^-21 |x|
+  static __device__ inline void split2(float a, float b, _Float16& ha, _Float16& hb, _Float16& la, _Float16& lb) {
+    const fp16x2 h = __builtin_amdgcn_cvt_pkrtz(a, b);
+    const fp16x2 l = __builtin_amdgcn_cvt_pkrtz(a - (float)h[0], b - (float)h[1]);
+    ha = (_Float16)h[0];
+    hb = (_Float16)h[1];
+    la = (_Float16)l[0];
+    lb = (_Float16)l[1];
   }
-  static __device__ inline f32x4 mfma(vec a, vec b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0); }
+  static __device__ inline f32x4 mfma(vec8 a, vec8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
 };
 template <> struct LPType<__bf16> {
-  typedef s16x4 vec;
+  typedef bf16x4 vec4;
+  typedef bf16x8 vec8;
   static __device__ inline float round(float x) { return (float)(__bf16)x; }
-  static __device__ inline short bits(float x) {
-    __bf16 h = (__bf16)x;
-    return *reinterpret_cast<short*>(&h);
+  static __device__ inline __bf16 cvt(float x) { return (__bf16)x; }
+  static __device__ inline void split2(float a, float b, __bf16& ha, __bf16& hb, __bf16& la, __bf16& lb) {
+    ha = (__bf16)a;
+    hb = (__bf16)b;
+    la = (__bf16)(a - (float)ha);
+    lb = (__bf16)(b - (float)hb);
   }
-  static __device__ inline vec pack(float a, float b, float c, float d) { return (vec){bits(a), bits(b), bits(c), bits(d)}; }
-  static __device__ inline f32x4 mfma(vec a, vec b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+  static __device__ inline f32x4 mfma(vec8 a, vec8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 };
 
-// 4 consecutive elements of row `row`, columns c0..c0+3 of a P x P slab (zero outside the slab)
+// K ordering of one 32-wide K tile t: lane group g = lane/16, element e (0..7) holds
+//   k = 32t + 4g + e            (e < 4)
+//   k = 32t + 16 + 4g + (e - 4) (e >= 4)
+// so each lane reads two aligned 4-element runs of a slab row, and the stage-A result (4 consecutive rows
+// per lane group of two 16-row tiles) is the stage-B A operand without any lane exchange.
+__host__ __device__ constexpr int lp_kidx(int t, int g, int e) { return 32 * t + (e < 4 ? 4 * g + e : 16 + 4 * g + (e - 4)); }
+
+// 4 consecutive elements of row `row`, columns c0..c0+3 of a P x P slab in LDS (zero outside the slab)
 template <typename T, int P>
-__device__ inline typename LPType<T>::vec slab4(const T* __restrict__ s, int row, int c0) {
-  typedef typename LPType<T>::vec V;
-  if ((P % 4) == 0) {  // c0 is a multiple of 4: the 4-vector is either fully inside or fully outside
-    const bool ok = row < P && c0 < P;
-    V v = *reinterpret_cast<const V*>(s + (ok ? row * P + c0 : 0));
-    const V z = {0, 0, 0, 0};
-    return ok ? v : z;
+__device__ inline typename LPType<T>::vec4 lds4(const T* s, int row, int c0) {
+  typedef typename LPType<T>::vec4 V;
+  if ((P % 4) == 0) {
+    // Read unconditionally: rows past P only feed output rows/columns that are never written, and
+    // columns past P meet zeros of the other operand; the buffers are padded and zero-filled so every
+    // such read stays inside the workgroup's LDS and finite.
+    return *reinterpret_cast<const V*>(s + row * P + c0);
   }
   V v;
-  for (int k = 0; k < 4; ++k) reinterpret_cast<T*>(&v)[k] = (row < P && c0 + k < P) ? s[row * P + c0 + k] : (T)0.0f;
+  for (int k = 0; k < 4; ++k) {
+    const bool ok = row < P && c0 + k < P;
+    const T e = s[ok ? row * P + c0 + k : 0];
+    v[k] = ok ? e : (T)0.0f;
+  }
   return v;
 }
+// B operand of K tile t for slab row `row`: elements e hold columns lp_kidx(t, lane/16, e)
+template <typename T, int P>
+__device__ inline typename LPType<T>::vec8 lds8(const T* s, int row, int t, int g) {
+  const typename LPType<T>::vec4 a = lds4<T, P>(s, row, 32 * t + 4 * g), b = lds4<T, P>(s, row, 32 * t + 16 + 4 * g);
+  return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7);
+}
 
-// SPLIT: the vector and the stage-B intermediate are carried as hi + lo pairs of the low-precision
-// type (two MFMAs each), so only the slabs are rounded once; without it every operand is rounded.
-// One wave owns a contiguous range of alpha and every p' tile; operands come straight from the
-// unpadded slabs in global memory (8-byte fragments, zero outside).  The 4 waves of a workgroup
-// reduce their Y through LDS in a fixed order.
-template <int P, int DP, typename T, bool SPLIT>
-__global__ void __launch_bounds__(256) lp_kernel(const LPPlan plan, const double* __restrict__ v,
-                                                 float* __restrict__ partial) {
+// Stage-A operand fragments of the vector, shared by every wave: V^T[idx][p] with idx = s*16*QT + q,
+// scaled by 1/max|v| and split into hi + lo parts of the low-precision type, in MFMA A-operand order:
+// frag[(m*KT + t)*64 + lane][e] = V^T[16m + lane%16][lp_kidx(t, lane/16, e)] (zero outside).
+// One workgroup; max|v| (fixed-order tree) goes to scale[0] for the final rescale.
+template <int P, int DP, typename T>
+__global__ void __launch_bounds__(1024) lp_prepare_kernel(const double* __restrict__ v, typename LPType<T>::vec8* __restrict__ fhi,
+                                                          typename LPType<T>::vec8* __restrict__ flo, double* __restrict__ scale) {
   typedef LPType<T> LT;
-  typedef typename LT::vec V;
-  constexpr int NT = (P + 15) / 16, KU = NT, Q16 = 16 * KU, MT = DP * KU;
-  constexpr int SLAB = P * P, NY = P * P * DP;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  __shared__ float red[4 * NY > 256 ? 4 * NY : 256];
-
-  // max|v| over the whole vector (fixed order) -> the vector is used as v / max|v|
-  {
-    float mx = 0.f;
-    for (int i = threadIdx.x; i < NY; i += 256) mx = fmaxf(mx, (float)fabs(v[i]));
-    red[threadIdx.x] = mx;
+  constexpr int QT = (P + 15) / 16, KT = (P + 31) / 32, MT = DP * QT, NY = P * P * DP;
+  __shared__ float red[1024];
+  float mx = 0.f;
+  for (int i = threadIdx.x; i < NY; i += 1024) mx = fmaxf(mx, (float)fabs(v[i]));
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int st = 512; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + st]);
     __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-      if ((int)threadIdx.x < st) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + st]);
-      __syncthreads();
+  }
+  const float m0 = red[0] > 0.f ? red[0] : 1.f;
+  const double vinv = 1.0 / (double)m0;
+  if (threadIdx.x == 0) scale[0] = (double)m0;
+  for (int f = threadIdx.x; f < MT * KT * 64; f += 1024) {
+    const int lane = f & 63, mt = f >> 6, m = mt / KT, t = mt % KT;
+    const int s = m / QT, q = 16 * (m % QT) + (lane & 15);
+    typename LT::vec8 hi, lo;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int p = lp_kidx(t, lane >> 4, e);
+      const float x = (q < P && p < P) ? (float)(v[(p * P + q) * DP + s] * vinv) : 0.0f;
+      hi[e] = LT::cvt(x);
+      lo[e] = LT::cvt(x - LT::round(x));
+    }
+    fhi[f] = hi;
+    flo[f] = lo;
+  }
+}
+
+// Direct global->LDS staging of one alpha's chunk slabs: wave w takes slots w, w + NT, ...; each
+// 64-lane instruction moves 64 consecutive 16-byte pieces of one slab (slot base pointers are
+// wave-uniform scalar loads, so no vector load sits between the issues).
+template <int P, int NT>
+__device__ __attribute__((always_inline)) inline void lp_stage(const LPChunk* __restrict__ C, int n_slots, int64_t al, uint4* buf, int wave, int lane) {
+  constexpr int SPB = lp_stride(P) * 2, CPS = SPB / 16;
+  for (int s = wave; s < n_slots; s += NT) {
+    const char* src = reinterpret_cast<const char*>(C->slot_ptr[s]) + al * SPB;
+#pragma unroll
+    for (int k = 0; k < (CPS + 63) / 64; ++k) {
+      const int j = 64 * k + lane;
+      if (j < CPS)
+        __builtin_amdgcn_global_load_lds(reinterpret_cast<const void*>(src + 16 * j), (lds_void*)(buf + s * CPS + 64 * k), 16, 0, 0);
     }
   }
-  const double vinv = 1.0 / (double)(red[0] > 0.f ? red[0] : 1.f);
-  __syncthreads();
+}
 
-  V vt[MT][KU], vl[MT][KU];
+// One alpha of one chunk for wave np (p' tile np).  Stage A: W tiles m = s*QT + mq over K tiles of p,
+// converted once per A run to hi + lo stage-B operands (element e < 4 from W tile s*QT + 2t, e >= 4 from
+// tile s*QT + 2t + 1, zero past QT).  Per term: Z[s] = W[s] B over K tiles of q, then
+// Y[s'] += sum_s C_t[s'][s] Z[s] (only the diagonal when C_t is diagonal), which keeps the per-term
+// vector work to the coefficient combine.
+template <int P, int DP, typename T, bool SPLIT>
+__device__ __attribute__((always_inline)) inline void lp_alpha(const LPDevPlan* __restrict__ plan, const LPChunk* __restrict__ C, int n_runs,
+                                const T* __restrict__ lds, const typename LPType<T>::vec8 (&vt)[DP * ((P + 15) / 16)][(P + 31) / 32],
+                                const typename LPType<T>::vec8 (&vl)[DP * ((P + 15) / 16)][(P + 31) / 32],
+                                f32x4 (&y)[(P + 15) / 16][DP], int np, int lane) {
+  typedef LPType<T> LT;
+  typedef typename LT::vec8 V;
+  constexpr int NT = (P + 15) / 16, QT = NT, KT = (P + 31) / 32, MT = DP * QT, SP = lp_stride(P);
+  const int g = lane >> 4;
+  for (int r = 0; r < n_runs; ++r) {
+    const T* As = lds + C->run_slot[r] * SP;
+    V whi[DP][KT], wlo[DP][KT];
+    {
+      f32x4 w[MT];
+#pragma unroll
+      for (int m = 0; m < MT; ++m) w[m] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int t = 0; t < KT; ++t) {
+        // hi products of all MT tiles, then the lo products: dependent MFMAs are MT issues apart
+        const V bop = lds8<T, P>(As, 16 * np + (lane & 15), t, g);
+#pragma unroll
+        for (int m = 0; m < MT; ++m) w[m] = LT::mfma(vt[m][t], bop, w[m]);
+        if (SPLIT) {
+#pragma unroll
+          for (int m = 0; m < MT; ++m) w[m] = LT::mfma(vl[m][t], bop, w[m]);
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < DP; ++s)
+#pragma unroll
+        for (int t = 0; t < KT; ++t)
+#pragma unroll
+          for (int e = 0; e < 8; e += 2) {
+            const int m = e < 4 ? 2 * t : 2 * t + 1;
+            const float x0 = m < QT ? w[s * QT + m][e & 3] : 0.0f, x1 = m < QT ? w[s * QT + m][(e & 3) + 1] : 0.0f;
+            T h0, h1, l0, l1;
+            LT::split2(x0, x1, h0, h1, l0, l1);
+            whi[s][t][e] = h0;
+            whi[s][t][e + 1] = h1;
+            wlo[s][t][e] = l0;
+            wlo[s][t][e + 1] = l1;
+          }
+    }
+    const int t1 = C->run_t0[r + 1];
+    for (int tt = C->run_t0[r]; tt < t1; ++tt) {
+      const T* Bs = lds + plan->term_slot[tt] * SP;
+      f32x4 z[NT][DP];
+#pragma unroll
+      for (int nq = 0; nq < NT; ++nq)
+#pragma unroll
+        for (int s = 0; s < DP; ++s) z[nq][s] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int t = 0; t < KT; ++t) {
+        V bop[NT];
+#pragma unroll
+        for (int nq = 0; nq < NT; ++nq) bop[nq] = lds8<T, P>(Bs, 16 * nq + (lane & 15), t, g);
+        // hi products of all NT*DP accumulators, then the lo products
+#pragma unroll
+        for (int nq = 0; nq < NT; ++nq)
+#pragma unroll
+          for (int s = 0; s < DP; ++s) z[nq][s] = LT::mfma(whi[s][t], bop[nq], z[nq][s]);
+        if (SPLIT) {
+#pragma unroll
+          for (int nq = 0; nq < NT; ++nq)
+#pragma unroll
+            for (int s = 0; s < DP; ++s) z[nq][s] = LT::mfma(wlo[s][t], bop[nq], z[nq][s]);
+        }
+      }
+      const float* cf = plan->coef[tt];
+      if (plan->term_diag[tt]) {
+#pragma unroll
+        for (int nq = 0; nq < NT; ++nq)
+#pragma unroll
+          for (int s = 0; s < DP; ++s) y[nq][s] += cf[s * DP + s] * z[nq][s];
+      } else {
+#pragma unroll
+        for (int nq = 0; nq < NT; ++nq)
+#pragma unroll
+          for (int sp = 0; sp < DP; ++sp)
+#pragma unroll
+            for (int s = 0; s < DP; ++s) y[nq][sp] += cf[sp * DP + s] * z[nq][s];
+      }
+    }
+  }
+}
+
+template <int P, int DP, typename T, bool SPLIT>
+__global__ void __launch_bounds__(64 * ((P + 15) / 16), 2) lp_kernel(const LPDevPlan* __restrict__ plan,
+                                                                     const typename LPType<T>::vec8* __restrict__ fhi,
+                                                                     const typename LPType<T>::vec8* __restrict__ flo,
+                                                                     float* __restrict__ partial) {
+  typedef typename LPType<T>::vec8 V;
+  constexpr int NT = (P + 15) / 16, KT = (P + 31) / 32, MT = DP * NT, NY = P * P * DP;
+  constexpr int CPS = lp_stride(P) * 2 / 16, BUF = lp_slots(P) * CPS + lp_pad(P);
+  __shared__ uint4 buf0[BUF];
+  __shared__ uint4 buf1[BUF];
+  const int lane = threadIdx.x & 63;
+  for (int i = threadIdx.x; i < BUF; i += 64 * NT) {
+    buf0[i] = uint4{0u, 0u, 0u, 0u};
+    buf1[i] = uint4{0u, 0u, 0u, 0u};
+  }
+  __syncthreads();
+  const int np = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const LPChunk* C = &plan->chunk[blockIdx.y];
+  const int n_slots = C->n_slots, n_runs = C->n_runs;
+
+  V vt[MT][KT], vl[MT][KT];
 #pragma unroll
   for (int m = 0; m < MT; ++m)
 #pragma unroll
-    for (int u = 0; u < KU; ++u) {
-      const int idx = 16 * m + (lane & 15), s = idx / Q16, q = idx % Q16;
-      float e[4], r[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int p = 16 * u + 4 * (lane >> 4) + k;
-        e[k] = (s < DP && q < P && p < P) ? (float)(v[(p * P + q) * DP + s] * vinv) : 0.0f;
-        r[k] = e[k] - LT::round(e[k]);
-      }
-      vt[m][u] = LT::pack(e[0], e[1], e[2], e[3]);
-      vl[m][u] = LT::pack(r[0], r[1], r[2], r[3]);
+    for (int t = 0; t < KT; ++t) {
+      vt[m][t] = fhi[(m * KT + t) * 64 + lane];
+      if (SPLIT) vl[m][t] = flo[(m * KT + t) * 64 + lane];
+      else vl[m][t] = V{};
     }
+  f32x4 y[NT][DP];
+#pragma unroll
+  for (int j = 0; j < NT; ++j)
+#pragma unroll
+    for (int s = 0; s < DP; ++s) y[j][s] = (f32x4){0.f, 0.f, 0.f, 0.f};
 
-  f32x4 y[NT][NT][DP];
-#pragma unroll
-  for (int i = 0; i < NT; ++i)
-#pragma unroll
-    for (int j = 0; j < NT; ++j)
-#pragma unroll
-      for (int s = 0; s < DP; ++s) y[i][j][s] = (f32x4){0.f, 0.f, 0.f, 0.f};
-
-  const int64_t gw = (int64_t)blockIdx.x * 4 + wave, nw = (int64_t)gridDim.x * 4;
-  const int64_t a0 = plan.n_alpha * gw / nw, a1 = plan.n_alpha * (gw + 1) / nw;
-  for (int64_t al = a0; al < a1; ++al) {
-    int t = 0;
-    while (t < plan.n_terms) {
-      const int a = plan.ta[t];
-      const T* As = reinterpret_cast<const T*>(plan.A[a]) + al * SLAB;
-      int t_end = t;
-      while (t_end < plan.n_terms && plan.ta[t_end] == a) ++t_end;
-#pragma unroll
-      for (int np = 0; np < NT; ++np) {
-        f32x4 w[MT];
-#pragma unroll
-        for (int m = 0; m < MT; ++m) w[m] = (f32x4){0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int u = 0; u < KU; ++u) {
-          const V bop = slab4<T, P>(As, 16 * np + (lane & 15), 16 * u + 4 * (lane >> 4));
-#pragma unroll
-          for (int m = 0; m < MT; ++m) {
-            w[m] = LT::mfma(vt[m][u], bop, w[m]);
-            if (SPLIT) w[m] = LT::mfma(vl[m][u], bop, w[m]);
-          }
-        }
-        for (int tt = t; tt < t_end; ++tt) {
-          const T* Bs = reinterpret_cast<const T*>(plan.B[plan.tb[tt]]) + al * SLAB;
-          const float* cf = plan.coef[tt];
-#pragma unroll
-          for (int u = 0; u < KU; ++u) {
-            V xo[DP], xl[DP];
-#pragma unroll
-            for (int sp = 0; sp < DP; ++sp) {
-              f32x4 acc = (f32x4){0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-              for (int s = 0; s < DP; ++s) acc += cf[sp * DP + s] * w[s * KU + u];
-              xo[sp] = LT::pack(acc[0], acc[1], acc[2], acc[3]);
-              xl[sp] = LT::pack(acc[0] - LT::round(acc[0]), acc[1] - LT::round(acc[1]), acc[2] - LT::round(acc[2]),
-                                acc[3] - LT::round(acc[3]));
-            }
-#pragma unroll
-            for (int nq = 0; nq < NT; ++nq) {
-              const V bop = slab4<T, P>(Bs, 16 * nq + (lane & 15), 16 * u + 4 * (lane >> 4));
-#pragma unroll
-              for (int sp = 0; sp < DP; ++sp) {
-                y[np][nq][sp] = LT::mfma(xo[sp], bop, y[np][nq][sp]);
-                if (SPLIT) y[np][nq][sp] = LT::mfma(xl[sp], bop, y[np][nq][sp]);
-              }
-            }
-          }
-        }
-      }
-      t = t_end;
-    }
-  }
-  // fixed-order workgroup reduction: wave w writes slot w, the slots are summed 0..3
-  __syncthreads();
-#pragma unroll
-  for (int np = 0; np < NT; ++np)
-#pragma unroll
-    for (int nq = 0; nq < NT; ++nq)
-#pragma unroll
-      for (int sp = 0; sp < DP; ++sp)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int pp = 16 * np + 4 * (lane >> 4) + r, qq = 16 * nq + (lane & 15);
-          if (pp < P && qq < P) red[wave * NY + (pp * P + qq) * DP + sp] = y[np][nq][sp][r];
-        }
-  __syncthreads();
-  float* out = partial + (int64_t)blockIdx.x * NY;
-  for (int i = threadIdx.x; i < NY; i += 256) out[i] = ((red[i] + red[NY + i]) + red[2 * NY + i]) + red[3 * NY + i];
-}
-
-constexpr int LP_SLICES = 16;
-// pass 1: slices[j][i] = sum over groups g = j, j + LP_SLICES, ... of partial[g][i]   (FP64 in a fixed order)
-__global__ void __launch_bounds__(256) lp_reduce1_kernel(int64_t groups, int64_t n, const float* __restrict__ partial,
-                                                         double* __restrict__ slices) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int j = blockIdx.y;
-  if (i >= n) return;
-  double acc = 0.0;
-  for (int64_t g = j; g < groups; g += LP_SLICES) acc += (double)partial[g * n + i];
-  slices[(int64_t)j * n + i] = acc;
-}
-// pass 2: y[i] = max|v| * sum_j slices[j][i]  (max|v| recomputed exactly as the main kernel does)
-__global__ void __launch_bounds__(256) lp_reduce2_kernel(int64_t n, const double* __restrict__ slices,
-                                                         const double* __restrict__ v, double* __restrict__ y) {
-  __shared__ float red[256];
-  float mx = 0.f;
-  for (int64_t k = threadIdx.x; k < n; k += 256) mx = fmaxf(mx, (float)fabs(v[k]));
-  red[threadIdx.x] = mx;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+  const int64_t a0 = plan->n_alpha * blockIdx.x / gridDim.x, a1 = plan->n_alpha * (blockIdx.x + 1) / gridDim.x;
+  if (a0 < a1) lp_stage<P, NT>(C, n_slots, a0, buf0, np, lane);
+  for (int64_t al = a0; al < a1; al += 2) {
+    // buffer 0 holds alpha al; everyone has finished reading buffer 1 (alpha al-1)
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     __syncthreads();
+    if (al + 1 < a1) lp_stage<P, NT>(C, n_slots, al + 1, buf1, np, lane);
+    lp_alpha<P, DP, T, SPLIT>(plan, C, n_runs, reinterpret_cast<const T*>(buf0), vt, vl, y, np, lane);
+    if (al + 1 >= a1) break;
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    __syncthreads();
+    if (al + 2 < a1) lp_stage<P, NT>(C, n_slots, al + 2, buf0, np, lane);
+    lp_alpha<P, DP, T, SPLIT>(plan, C, n_runs, reinterpret_cast<const T*>(buf1), vt, vl, y, np, lane);
   }
-  const double scale = red[0] > 0.f ? (double)red[0] : 1.0;
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  double t = 0.0;
-  for (int j = 0; j < LP_SLICES; ++j) t += slices[(int64_t)j * n + i];
-  y[i] = t * scale;
+  // rows p' = 16 np + 4(lane/16) + r, columns q' = 16 nq + lane%16 of this workgroup's partial
+  float* out = partial + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * NY;
+#pragma unroll
+  for (int nq = 0; nq < NT; ++nq)
+#pragma unroll
+    for (int sp = 0; sp < DP; ++sp)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int pp = 16 * np + 4 * (lane >> 4) + r, qq = 16 * nq + (lane & 15);
+        if (pp < P && qq < P) out[(pp * P + qq) * DP + sp] = y[nq][sp][r];
+      }
+
 }
+
+// y[i] = max|v| * sum over partial rows in row order (FP64).  One workgroup owns 32 columns; its 8 row
+// groups (32 lanes each) sum rows g, g + 8, ... and the 8 group sums are added in group order.
+constexpr int LP_RCOLS = 32, LP_RGROUPS = 8;
+__global__ void __launch_bounds__(LP_RCOLS * LP_RGROUPS) lp_reduce_kernel(int64_t rows, int64_t n, const float* __restrict__ partial,
+                                                                         const double* __restrict__ scale, double* __restrict__ y) {
+  __shared__ double part[LP_RGROUPS][LP_RCOLS];
+  const int col = threadIdx.x % LP_RCOLS, grp = threadIdx.x / LP_RCOLS;
+  const int64_t i = (int64_t)blockIdx.x * LP_RCOLS + col;
+  double acc = 0.0;
+  if (i < n) {
+    int64_t r = grp;
+    for (; r + 3 * LP_RGROUPS < rows; r += 4 * LP_RGROUPS) {
+      const float a = partial[r * n + i], b = partial[(r + LP_RGROUPS) * n + i];
+      const float c = partial[(r + 2 * LP_RGROUPS) * n + i], d = partial[(r + 3 * LP_RGROUPS) * n + i];
+      acc += (double)a;
+      acc += (double)b;
+      acc += (double)c;
+      acc += (double)d;
+    }
+    for (; r < rows; r += LP_RGROUPS) acc += (double)partial[r * n + i];
+  }
+  part[grp][col] = acc;
+  __syncthreads();
+  if (grp == 0 && i < n) {
+    double t = 0.0;
+    for (int k = 0; k < LP_RGROUPS; ++k) t += part[k][col];
+    y[i] = t * scale[0];
+  }
+}
+
 
 // low-precision slab = slab / max|slab|: pass 1 per-block maxima, pass 2 (one block) their maximum
 __global__ void __launch_bounds__(256) lp_blockmax_kernel(int64_t n, const double* __restrict__ x, double* __restrict__ part) {
@@ -244,32 +393,61 @@ __global__ void __launch_bounds__(1024) lp_absmax_kernel(int64_t n, const double
   }
   if (threadIdx.x == 0) out[0] = red[0] > 0.0 ? red[0] : 1.0;
 }
+// y[alpha][i] = x[alpha][i] / scale for i < P*P, zero in the stride padding
 template <typename T>
-__global__ void lp_convert_kernel(int64_t n, const double* __restrict__ x, const double* __restrict__ scale, T* __restrict__ y) {
+__global__ void lp_convert_kernel(int64_t n_alpha, int64_t PP, int64_t SP, const double* __restrict__ x,
+                                  const double* __restrict__ scale, T* __restrict__ y) {
   const double inv = 1.0 / scale[0];
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    y[i] = (T)(x[i] * inv);
+  const int64_t total = n_alpha * SP;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t al = k / SP, i = k - al * SP;
+    y[k] = i < PP ? (T)(x[al * PP + i] * inv) : (T)0.0f;
+  }
+}
+
+// Workspace layout (in floats): device plan | partials [rows][n] | vector fragments hi, lo [MT*KT*64]
+// 16-byte vectors each | scale (one double).  Pieces start on 16-byte boundaries.
+struct LPLayout {
+  int64_t plan, partial, fhi, flo, scale, total;
+};
+inline int64_t lp_align4(int64_t x) { return (x + 3) & ~3LL; }
+inline LPLayout lp_layout(int64_t P, int64_t d, int64_t rows) {
+  const int64_t n = P * P * d, frags = d * ((P + 15) / 16) * ((P + 31) / 32) * 64;
+  LPLayout L;
+  L.plan = 0;
+  L.partial = lp_align4((int64_t)(sizeof(LPDevPlan) + 3) / 4);
+  L.fhi = lp_align4(L.partial + rows * n);
+  L.flo = L.fhi + 4 * frags;
+  L.scale = L.flo + 4 * frags;
+  L.total = L.scale + 4;
+  return L;
 }
 
 struct LPHandle {
-  LPPlan plan;
   int64_t P, d;
-  int dtype, groups, split;
-  float* work;  // groups * P*P*d partials + n scaled vector + 1 scale
+  int dtype, groups, chunks, split;
+  float* work;  // lp_layout(P, d, groups * LP_MAX_CHUNKS); the device plan sits at its start
 };
 
 template <int P, int DP, typename T>
 int launch_lp(hipStream_t st, const LPHandle* h, const double* v, double* y) {
   constexpr int NT = (P + 15) / 16;
+  typedef typename LPType<T>::vec8 V;
   const int64_t n = (int64_t)P * P * DP;
-  float* partial = h->work;
-  double* slices = reinterpret_cast<double*>(partial + (((int64_t)h->groups * n + 1) & ~1LL));
-  if (h->split) lp_kernel<P, DP, T, true><<<h->groups, 256, 0, st>>>(h->plan, v, partial);
-  else lp_kernel<P, DP, T, false><<<h->groups, 256, 0, st>>>(h->plan, v, partial);
+  const LPLayout L = lp_layout(P, DP, (int64_t)h->groups * LP_MAX_CHUNKS);
+  const LPDevPlan* plan = reinterpret_cast<const LPDevPlan*>(h->work + L.plan);
+  float* partial = h->work + L.partial;
+  V* fhi = reinterpret_cast<V*>(h->work + L.fhi);
+  V* flo = reinterpret_cast<V*>(h->work + L.flo);
+  double* scale = reinterpret_cast<double*>(h->work + L.scale);
+  lp_prepare_kernel<P, DP, T><<<1, 1024, 0, st>>>(v, fhi, flo, scale);
   CC_LAUNCH_CHECK();
-  lp_reduce1_kernel<<<dim3((unsigned)((n + 255) / 256), LP_SLICES), 256, 0, st>>>(h->groups, n, partial, slices);
+  const dim3 grid((unsigned)h->groups, (unsigned)h->chunks);
+  if (h->split) lp_kernel<P, DP, T, true><<<grid, 64 * NT, 0, st>>>(plan, fhi, flo, partial);
+  else lp_kernel<P, DP, T, false><<<grid, 64 * NT, 0, st>>>(plan, fhi, flo, partial);
   CC_LAUNCH_CHECK();
-  lp_reduce2_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, slices, v, y);
+  lp_reduce_kernel<<<(unsigned)((n + LP_RCOLS - 1) / LP_RCOLS), LP_RCOLS * LP_RGROUPS, 0, st>>>(
+      (int64_t)h->groups * h->chunks, n, partial, scale, y);
   CC_LAUNCH_CHECK();
   return CC_OK;
 }
@@ -286,6 +464,81 @@ int dispatch_lp(hipStream_t st, const LPHandle* h, const double* v, double* y) {
   cc_set_last_error("low-precision stage-3: shape not instantiated");
   return CC_ERR_UNSUPPORTED;
 }
+
+// Host: cut the (A-sorted) term table into chunks whose distinct slabs fit one LDS buffer.
+int lp_build_plan(LPDevPlan* dp, int64_t P, int64_t d, const void* const* A, const void* const* B, int n_terms,
+                  const int* ta, const int* tb, const float* coef, int64_t n_alpha) {
+  const int cap = lp_slots((int)P);
+  std::memset(dp, 0, sizeof(*dp));
+  dp->n_alpha = n_alpha;
+  int chunk = -1, out_t = 0;
+  std::vector<const void*> slots;
+  auto slot_of = [&](const void* ptr) {
+    for (size_t i = 0; i < slots.size(); ++i)
+      if (slots[i] == ptr) return (int)i;
+    return -1;
+  };
+  auto open_chunk = [&]() -> bool {
+    if (chunk + 1 >= LP_MAX_CHUNKS) return false;
+    ++chunk;
+    slots.clear();
+    dp->chunk[chunk].n_runs = 0;
+    dp->chunk[chunk].run_t0[0] = out_t;
+    return true;
+  };
+  if (!open_chunk()) return CC_ERR_UNSUPPORTED;
+  int t = 0;
+  while (t < n_terms) {
+    const void* a_ptr = A[ta[t]];
+    // terms of this A channel
+    int t_end = t;
+    while (t_end < n_terms && ta[t_end] == ta[t]) ++t_end;
+    // place as many of them as fit; open a new chunk when even the first does not fit
+    bool run_open = false;
+    for (int k = t; k < t_end; ++k) {
+      const void* b_ptr = B[tb[k]];
+      const int need = (slot_of(a_ptr) < 0 ? 1 : 0) + (slot_of(b_ptr) < 0 ? 1 : 0);
+      if ((int)slots.size() + need > cap) {
+        // close the current run and chunk
+        if (run_open) {
+          LPChunk& c = dp->chunk[chunk];
+          c.run_t0[c.n_runs + 1] = out_t;
+          ++c.n_runs;
+          run_open = false;
+        }
+        dp->chunk[chunk].n_slots = (int)slots.size();
+        for (size_t i = 0; i < slots.size(); ++i) dp->chunk[chunk].slot_ptr[i] = slots[i];
+        if (!open_chunk()) return CC_ERR_UNSUPPORTED;
+      }
+      if (!run_open) {
+        if (slot_of(a_ptr) < 0) slots.push_back(a_ptr);
+        LPChunk& c = dp->chunk[chunk];
+        c.run_slot[c.n_runs] = slot_of(a_ptr);
+        c.run_t0[c.n_runs] = out_t;
+        run_open = true;
+      }
+      if (slot_of(b_ptr) < 0) slots.push_back(b_ptr);
+      if ((int)slots.size() > cap) return CC_ERR_UNSUPPORTED;  // cap < 2
+      dp->term_slot[out_t] = slot_of(b_ptr);
+      dp->term_diag[out_t] = 1;
+      for (int e = 0; e < d * d; ++e) {
+        dp->coef[out_t][e] = coef[k * d * d + e];
+        if (e / d != e % d && coef[k * d * d + e] != 0.0f) dp->term_diag[out_t] = 0;
+      }
+      ++out_t;
+    }
+    if (run_open) {
+      LPChunk& c = dp->chunk[chunk];
+      c.run_t0[c.n_runs + 1] = out_t;
+      ++c.n_runs;
+    }
+    t = t_end;
+  }
+  dp->chunk[chunk].n_slots = (int)slots.size();
+  for (size_t i = 0; i < slots.size(); ++i) dp->chunk[chunk].slot_ptr[i] = slots[i];
+  dp->n_chunks = chunk + 1;
+  return CC_OK;
+}
 }  // namespace
 
 int cc_internal_lp_apply(hipStream_t st, const void* plan, const double* x, double* y) {
@@ -297,22 +550,32 @@ int64_t cc_internal_lp_n(const void* plan) {
   return h->P * h->P * h->d;
 }
 
-extern "C" int cc_stage3_lp_convert(void* stream, int dtype, int64_t n, const double* slab, double* scale_out, void* out) {
+extern "C" int cc_stage3_lp_slab_size(int64_t n_alpha, int64_t P, int64_t* out) {
+  if (n_alpha < 0 || P < 1) { cc_set_last_error("lp slab size: bad arguments"); return CC_ERR_ARG; }
+  *out = n_alpha * lp_stride((int)P);
+  return CC_OK;
+}
+
+extern "C" int cc_stage3_lp_convert(void* stream, int dtype, int64_t n_alpha, int64_t P, const double* slab,
+                                    double* scale_out, void* out) {
   hipStream_t st = cc_stream(stream);
-  if (n <= 0) return CC_OK;
+  if (n_alpha <= 0) return CC_OK;
+  const int64_t n = n_alpha * P * P, SP = lp_stride((int)P);
   lp_blockmax_kernel<<<512, 256, 0, st>>>(n, slab, scale_out + 1);   // scale_out holds 513 doubles
   CC_LAUNCH_CHECK();
   lp_absmax_kernel<<<1, 1024, 0, st>>>(512, scale_out + 1, scale_out);
   CC_LAUNCH_CHECK();
-  if (dtype == CC_F16) lp_convert_kernel<_Float16><<<cc_blocks(n, 256), 256, 0, st>>>(n, slab, scale_out, (_Float16*)out);
-  else if (dtype == CC_BF16) lp_convert_kernel<__bf16><<<cc_blocks(n, 256), 256, 0, st>>>(n, slab, scale_out, (__bf16*)out);
+  const unsigned blocks = cc_blocks(n_alpha * SP, 256);
+  if (dtype == CC_F16) lp_convert_kernel<_Float16><<<blocks, 256, 0, st>>>(n_alpha, P * P, SP, slab, scale_out, (_Float16*)out);
+  else if (dtype == CC_BF16) lp_convert_kernel<__bf16><<<blocks, 256, 0, st>>>(n_alpha, P * P, SP, slab, scale_out, (__bf16*)out);
   else { cc_set_last_error("lp convert: dtype must be f16 or bf16"); return CC_ERR_ARG; }
   CC_LAUNCH_CHECK();
   return CC_OK;
 }
 
 extern "C" int cc_stage3_lp_work_size(int64_t n_alpha, int64_t P, int64_t d, int groups, int64_t* out) {
-  *out = (int64_t)groups * P * P * d + 2 * LP_SLICES * P * P * d + 16;  // floats (the slices are doubles)
+  (void)n_alpha;
+  *out = lp_layout(P, d, (int64_t)groups * LP_MAX_CHUNKS).total;  // floats
   return CC_OK;
 }
 
@@ -328,32 +591,40 @@ extern "C" int cc_stage3_lp_plan_create(void** out, int dtype, int n_a, const vo
     cc_set_last_error("lp plan: exceeds kernel limits");
     return CC_ERR_UNSUPPORTED;
   }
-
-  LPHandle* h = new LPHandle();
-  h->plan = LPPlan{};
-  for (int i = 0; i < n_a; ++i) h->plan.A[i] = A[i];
-  for (int i = 0; i < n_b; ++i) h->plan.B[i] = B[i];
-  h->plan.n_a = n_a;
-  h->plan.n_b = n_b;
-  h->plan.n_terms = n_terms;
-  for (int t = 0; t < n_terms; ++t) {
+  for (int t = 0; t < n_terms; ++t)
     if (term_a[t] < 0 || term_a[t] >= n_a || term_b[t] < 0 || term_b[t] >= n_b || (t > 0 && term_a[t] < term_a[t - 1])) {
-      delete h;
       cc_set_last_error("lp plan: bad term table (indices out of range or not sorted by A channel)");
       return CC_ERR_ARG;
     }
-    h->plan.ta[t] = term_a[t];
-    h->plan.tb[t] = term_b[t];
-    for (int e = 0; e < d * d; ++e) h->plan.coef[t][e] = term_coef[t * d * d + e];
+  LPDevPlan* dp = new LPDevPlan();
+  int rc = lp_build_plan(dp, P, d, A, B, n_terms, term_a, term_b, term_coef, n_alpha);
+  if (rc != CC_OK) {
+    delete dp;
+    cc_set_last_error("lp plan: the term table needs more chunks than the kernel supports");
+    return rc;
   }
-  h->plan.n_alpha = n_alpha;
+  const hipError_t e = hipMemcpy(work, dp, sizeof(LPDevPlan), hipMemcpyHostToDevice);
+  LPHandle* h = new LPHandle();
   h->P = P;
   h->d = d;
   h->dtype = dtype;
   h->groups = groups;
+  h->chunks = dp->n_chunks;
   h->split = split ? 1 : 0;
   h->work = work;
+  delete dp;
+  if (e != hipSuccess) {
+    delete h;
+    cc_set_last_error(hipGetErrorString(e));
+    return CC_ERR_HIP;
+  }
   *out = h;
+  return CC_OK;
+}
+
+extern "C" int cc_stage3_lp_plan_chunks(const void* plan, int* out) {
+  if (!plan) { cc_set_last_error("null plan"); return CC_ERR_ARG; }
+  *out = static_cast<const LPHandle*>(plan)->chunks;
   return CC_OK;
 }
 

@@ -42,9 +42,10 @@ class LowPrecisionOperator:
         rows.sort(key=lambda r: (r[0], r[1]))
         self._slabs, scales = [], []
         sc = empty((len(a_list) + len(b_list), 513))
+        slab_elems = _lib.query("cc_stage3_lp_slab_size", self.n_alpha, self.P)
         for i, ch in enumerate(a_list + b_list):
-            buf = torch.empty(ch.size(), dtype=torch.float16 if dtype == "f16" else torch.bfloat16, device=sc.device)
-            _lib.call("cc_stage3_lp_convert", stream(), code, ch.size(), ptr(ch._t), ptr(sc[i]), ptr(buf))
+            buf = torch.empty(slab_elems, dtype=torch.float16 if dtype == "f16" else torch.bfloat16, device=sc.device)
+            _lib.call("cc_stage3_lp_convert", stream(), code, self.n_alpha, self.P, ptr(ch._t), ptr(sc[i]), ptr(buf))
             self._slabs.append(buf)
         scales = sc[:, 0].tolist()
         sa, sb = scales[:len(a_list)], scales[len(a_list):]
@@ -52,7 +53,7 @@ class LowPrecisionOperator:
         for (a, _, b, O) in rows:
             m = np.identity(self.d) if O is None else O.toArray()
             coef.extend((sa[a] * sb[b] * m).ravel().tolist())
-        self.groups = int(groups if groups is not None else max(1, min((self.n_alpha + 15) // 16, 256)))
+        self.groups = int(groups if groups is not None else max(1, min(self.n_alpha // 4, 2048)))
         self._work = torch.empty(_lib.query("cc_stage3_lp_work_size", self.n_alpha, self.P, self.d, self.groups),
                                  dtype=torch.float32, device=sc.device)
         self._ptrs_a = _lib.ptr_array([t.data_ptr() for t in self._slabs[:len(a_list)]])
@@ -62,7 +63,10 @@ class LowPrecisionOperator:
                   len(rows), _lib.i32_array([r[0] for r in rows]), _lib.i32_array([r[2] for r in rows]),
                   _lib.f32_array(coef), self.n_alpha, self.P, self.d, self.groups, 1 if split else 0, ptr(self._work))
         self.plan = h
-        self.bytes_per_matvec = sum(t.numel() * t.element_size() for t in self._slabs) + 2 * self.n * 8
+        chunks = ctypes.c_int(0)
+        _lib.call("cc_stage3_lp_plan_chunks", h, ctypes.byref(chunks))
+        self.chunks = chunks.value
+        self.bytes_per_matvec = len(self._slabs) * self.n_alpha * self.P * self.P * 2 + 2 * self.n * 8
 
     def apply_flat(self, x, y):
         _lib.call("cc_stage3_lp_plan_apply", stream(), self.plan, ptr(x), ptr(y))

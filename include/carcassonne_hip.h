@@ -169,16 +169,23 @@ int cc_lanczos_f64(void* stream, const void* plan, int64_t n_env, int64_t d, con
 
 /* ---- precision form: FP16 / BF16 stage-3 matvec with FP32 accumulation ---------------
  * (no reference counterpart; the same arithmetic as formDenseStage3 at reduced precision)
- * cc_stage3_lp_convert: out = slab / max|slab| rounded to dtype; scale_out[0] = max|slab|.
- * cc_stage3_lp_plan_create: slabs stored unpadded (n_alpha, P, P); term_coef[t] is the d x d
- * matrix scale_a * scale_b * O_t (identity for a normalization-type term); split != 0 carries the
- * vector and the intermediate as hi + lo pairs (only the slabs are rounded); `work` holds
- * cc_stage3_lp_work_size floats.  Apply computes y (FP64) = H_lp x (FP64). */
-int cc_stage3_lp_convert(void* stream, int dtype, int64_t n, const double* slab, double* scale_out, void* out);
+ * Low-precision slabs are stored (n_alpha, P, P) with the per-alpha stride rounded up to a multiple
+ * of 8 elements (zero padding); cc_stage3_lp_slab_size gives the element count of one channel.
+ * cc_stage3_lp_convert: out = slab / max|slab| rounded to dtype (slab is the dense FP64 (n_alpha, P, P)
+ * channel); scale_out (513 doubles of scratch) receives max|slab| in scale_out[0].
+ * cc_stage3_lp_plan_create: term_coef[t] is the d x d matrix scale_a * scale_b * O_t (identity for a
+ * normalization-type term); split != 0 carries the vector and the intermediate as hi + lo pairs (only
+ * the slabs are rounded); `work` holds cc_stage3_lp_work_size floats and receives the device copy of
+ * the plan.  The terms are cut into chunks whose slabs fit the kernel's LDS buffers;
+ * cc_stage3_lp_plan_chunks reports how many.  Apply computes y (FP64) = H_lp x (FP64). */
+int cc_stage3_lp_slab_size(int64_t n_alpha, int64_t P, int64_t* out);
+int cc_stage3_lp_convert(void* stream, int dtype, int64_t n_alpha, int64_t P, const double* slab, double* scale_out,
+                         void* out);
 int cc_stage3_lp_work_size(int64_t n_alpha, int64_t P, int64_t d, int groups, int64_t* out);
 int cc_stage3_lp_plan_create(void** out, int dtype, int n_a, const void* const* A, int n_b, const void* const* B,
                              int n_terms, const int* term_a, const int* term_b, const float* term_coef,
                              int64_t n_alpha, int64_t P, int64_t d, int groups, int split, float* work);
+int cc_stage3_lp_plan_chunks(const void* plan, int* out);
 int cc_stage3_lp_plan_apply(void* stream, const void* plan, const double* x, double* y);
 int cc_stage3_lp_plan_destroy(void* plan);
 
