@@ -1,12 +1,17 @@
 #!/usr/bin/env python
 """Benchmark: ground-state solves/s of the center-site effective Hamiltonian on one MI355X.
 
-Workload (BASELINE precision-form shape): 2D transverse-field Ising model H = -sum ZZ - 3.0 sum X,
-center bond D = 6, fused environment bond chi = 64, d = 2, w = 3 operator channels on the sides;
-seeded synthetic environment (carcassonne_amd.synthetic, seed 0) resident in HBM before timing.
-One step = one solve: stage 1/2 on the GPU, normalization factor (N_env = L L^T, W = L^-1),
-FP64 Lanczos to a 1e-10 relative residual (or a Ritz-value change below 1e-13), final FP64
-Rayleigh quotient; the energy is checked against the oracle's eigh energy (tests/golden).
+Workload: 2D transverse-field Ising model H = -sum ZZ - 3.0 sum X, center bond D = 6, fused
+environment bond chi = 64, d = 2, w = 3 operator channels on the sides; seeded synthetic environment
+(carcassonne_amd.synthetic, seed 0) resident in HBM before timing.  One step = one solve:
+stage 1/2 on the GPU, the stage-3 operators, the normalization factor (N_env = L L^T, W = L^-1), the
+eigensolve to a 1e-10 relative energy, and the FP64 Rayleigh quotient of the result; the energy is
+checked against the oracle's eigh energy (tests/golden).
+
+Two forms are timed on the same inputs:
+  * mixed-f16 (the headline `value`): FP16 stage-3 matvecs with FP32 accumulation inside the Lanczos,
+    FP64 Jacobi-Davidson refinement, FP64 Rayleigh-Ritz energy;
+  * fp64: the plain FP64 Lanczos form (`plain_fp64_*` fields).
 """
 import argparse
 import json
@@ -19,6 +24,7 @@ sys.path.insert(0, ROOT)
 
 WORKLOAD = dict(D=6, chi=64, d=2, model="ising", h=3.0, seed=0)
 PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
 
 
 def golden_energy():
@@ -31,7 +37,7 @@ def golden_energy():
 def cpu_baseline(gpu_iterations, sample_iterations=6):
     """The oracle (NumPy/SciPy FP64) on the same workload, timed on this host: stage 1/2,
     normalization factor, and `sample_iterations` Lanczos steps; a solve is extrapolated to the
-    GPU's iteration count."""
+    plain FP64 GPU form's iteration count."""
     import numpy as np
     from scipy.linalg import cholesky, solve_triangular
     from threadpoolctl import threadpool_info
@@ -62,15 +68,17 @@ def cpu_baseline(gpu_iterations, sample_iterations=6):
     threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
     return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": int(threads), "kind": "port",
             "sample": "oracle stage 1/2 + Cholesky of N_env + {} Lanczos steps at D=6, chi=64 ({:.1f} s); solve "
-                      "time extrapolated to the GPU's {} iterations".format(sample_iterations, t3 - t0, gpu_iterations),
+                      "time extrapolated to the plain FP64 form's {} iterations".format(sample_iterations, t3 - t0,
+                                                                                        gpu_iterations),
             "host_cpus": len(os.sched_getaffinity(0))}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--form", choices=["both", "mixed", "fp64"], default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -96,44 +104,80 @@ def main():
     op_center = {tag(k): HipData.fromArray(v) for k, v in opc.items()}
     x0 = HipData.fromArray(st)
     synchronize()
-
-    for _ in range(args.warmup):
-        r = pipeline.solve(corners, sides, op_center, x0)
-    synchronize()
-    _lib.load().cc_profile_begin()
-    t0 = time.perf_counter()
-    results = []
-    for _ in range(args.steps):
-        results.append(pipeline.solve(corners, sides, op_center, x0))
-    synchronize()
-    t1 = time.perf_counter()
-    kms, launches = _lib.f64_array([0]), _lib.i64_array([0])
-    _lib.call("cc_profile_end", kms, launches)
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
     e_ref = golden_energy()
-    r = results[-1]
-    err = abs(r["energy"] - e_ref) / abs(e_ref)
-    op = r["operator"]
-    flops = pipeline.flops_per_matvec(op)
-    kernel_ms = kms[0] / max(1, launches[0])
-    achieved = flops / (kernel_ms * 1e-3) / 1e12
+
+    def run(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        synchronize()
+        _lib.load().cc_profile_begin()
+        t0 = time.perf_counter()
+        results = [fn() for _ in range(steps)]
+        synchronize()
+        t1 = time.perf_counter()
+        kms, launches = _lib.f64_array([0, 0]), _lib.i64_array([0, 0])
+        _lib.call("cc_profile_end", 2, kms, launches)
+        return (t1 - t0) * 1e3 / steps, results[-1], list(kms), list(launches)
+
+    out = {}
+    if args.form in ("both", "fp64"):
+        ms, r, kms, nl = run(lambda: pipeline.solve(corners, sides, op_center, x0), args.steps, args.warmup)
+        op = r["operator"]
+        flops = pipeline.flops_per_matvec(op)
+        kernel_ms = kms[0] / max(1, nl[0])
+        out["fp64"] = dict(ms=ms, energy=r["energy"], err=abs(r["energy"] - e_ref) / abs(e_ref),
+                           iterations=r["iterations"], kernel_ms=kernel_ms, launches=nl[0], flops=flops,
+                           tflops=flops / (kernel_ms * 1e-3) / 1e12, terms=len(op._A))
+    if args.form in ("both", "mixed"):
+        ms, r, kms, nl = run(lambda: pipeline.solve_mixed(corners, sides, op_center, x0, dtype="f16"),
+                             args.steps, args.warmup)
+        lp = r["lp_operator"]
+        kernel_ms = kms[1] / max(1, nl[1])
+        lp_bytes = lp.bytes_per_matvec
+        out["mixed"] = dict(ms=ms, energy=r["energy"], err=abs(r["energy"] - e_ref) / abs(e_ref),
+                            iterations=r["iterations"], info=r["info"], kernel_ms=kernel_ms, launches=nl[1],
+                            fp64_kernel_ms=kms[0] / max(1, nl[0]), fp64_launches=nl[0], bytes=lp_bytes,
+                            gbs=lp_bytes / (kernel_ms * 1e-3) / 1e9, flops=pipeline.flops_per_matvec(r["operator"]))
+
+    head = out.get("mixed") or out["fp64"]
+    form = "mixed-f16" if "mixed" in out else "fp64-lanczos"
     line = {
         "metric": "ground-state solves/sec at D=6,chi=64 to 1e-10 rel. energy vs plain FP64 GPU",
-        "value": 1e3 / ms_per_step, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic (seeded environment, carcassonne_amd.synthetic seed 0)",
-        "config": {"workload": "2D TFIM h=3.0, D=6, chi=64, d=2, w=3; plain FP64 Lanczos form",
-                   "form": "fp64-lanczos", "D": 6, "chi": 64, "terms": len(op._A)},
-        "energy": r["energy"], "energy_rel_err_vs_oracle": err, "lanczos_iterations": r["iterations"],
-        "plain_fp64_solves_per_s": 1e3 / ms_per_step,
-        "matvec_kernel_ms": kernel_ms, "matvec_launches_timed": int(launches[0]),
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel": "fused_f64_kernel<36,2> (stage-3 matvec)",
-                     "algorithmic_flops_per_launch": flops},
+        "value": 1e3 / head["ms"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": head["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f16+f64" if form == "mixed-f16" else "f64",
+        "data": "synthetic (seeded environment, carcassonne_amd.synthetic seed 0)",
+        "config": {"workload": "2D TFIM h=3.0, D=6, chi=64, d=2, w=3; one solve = stage 1/2 + normalization "
+                               "factor + eigensolve to 1e-10 rel. energy + FP64 Rayleigh quotient",
+                   "form": form, "D": 6, "chi": 64},
+        "energy": head["energy"], "energy_rel_err_vs_oracle": head["err"],
     }
+    if "fp64" in out:
+        f = out["fp64"]
+        line.update(plain_fp64_solves_per_s=1e3 / f["ms"], plain_fp64_ms_per_solve=f["ms"],
+                    plain_fp64_energy_rel_err=f["err"], plain_fp64_lanczos_iterations=f["iterations"],
+                    plain_fp64_matvec_kernel_ms=f["kernel_ms"], plain_fp64_matvec_tflops=f["tflops"],
+                    plain_fp64_matvec_frac_of_fp64_peak=f["tflops"] / PEAK_FP64_TFLOPS)
+    if "mixed" in out:
+        m = out["mixed"]
+        line.update(mixed_lowprec_iterations=m["iterations"], mixed_info=m["info"],
+                    lp_matvec_kernel_ms=m["kernel_ms"], lp_matvec_launches_timed=m["launches"],
+                    lp_matvec_effective_tflops=m["flops"] / (m["kernel_ms"] * 1e-3) / 1e12)
+        if "fp64" in out:
+            line["speedup_vs_plain_fp64"] = out["fp64"]["ms"] / m["ms"]
+        line["roofline"] = {"bound": "hbm", "achieved": m["gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                            "frac": m["gbs"] / PEAK_HBM_GBS, "traffic": None,
+                            "kernel": "lp_kernel<36,2,f16,split> (stage-3 matvec, FP16 slabs)",
+                            "algorithmic_bytes_per_launch": m["bytes"]}
+    else:
+        f = out["fp64"]
+        line["roofline"] = {"bound": "mfma", "achieved": f["tflops"], "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                            "frac": f["tflops"] / PEAK_FP64_TFLOPS, "traffic": None,
+                            "kernel": "fused_f64_kernel<36,2> (stage-3 matvec)",
+                            "algorithmic_flops_per_launch": f["flops"]}
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(r["iterations"])
+        its = out["fp64"]["iterations"] if "fp64" in out else 80
+        line["cpu_baseline"] = cpu_baseline(its)
     print(json.dumps(line), flush=True)
 
 

@@ -46,7 +46,7 @@ SIGNATURES = {
     "cc_stage3_plan_apply": [_vp, _vp, _vp, _vp],
     "cc_stage3_plan_destroy": [_vp],
     "cc_profile_begin": [],
-    "cc_profile_end": [_pf64, _pi64],
+    "cc_profile_end": [_i32, _pf64, _pi64],
     "cc_linalg_work_size": [_i64, _pi64],
     "cc_cholesky_f64": [_vp, _i64, _vp, _vp, _vp],
     "cc_trtri_lower_f64": [_vp, _i64, _vp, _vp, _vp],
@@ -68,6 +68,12 @@ SIGNATURES = {
     "cc_small_geneig_f64": [_vp, _i32, _vp, _vp, _vp, _vp],
     "cc_jd_work_size": [_i64, _pi64],
     "cc_jd_correction_lp": [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _f64, _vp, _vp, _i32, _vp, _vp],
+    "cc_mlp_param_count": [_i64, _i64, _i64, _pi64],
+    "cc_mlp_work_size": [_i64, _i64, _i64, _pi64],
+    "cc_mlp_features": [_vp, _i32, _i32, _vp, _f64, _i64, _vp],
+    "cc_mlp_forward": [_vp, _i64, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
+    "cc_mlp_backward": [_vp, _i64, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "cc_mlp_adam": [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _f64, _f64, _f64, _f64, _i64],
     "cc_stage3_matvec_fused_f64": [_vp, _i32, _pvp, _i32, _pvp, _i32, _pi32, _pi32, _pi32, _i32, _pf64, _i64, _i64,
                                    _i64, _vp, _vp, _vp],
 }

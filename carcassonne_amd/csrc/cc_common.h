@@ -44,3 +44,9 @@ int cc_internal_stage3_apply(hipStream_t st, const void* plan, const double* x, 
 int64_t cc_internal_stage3_n(const void* plan);
 int cc_internal_lp_apply(hipStream_t st, const void* plan, const double* x, double* y);
 int64_t cc_internal_lp_n(const void* plan);
+
+// Launch timing (between cc_profile_begin and cc_profile_end, single-threaded use): a kernel launch is
+// bracketed by cc_prof_start / cc_prof_stop, which record HIP events on its stream when timing is on.
+enum { CC_PROF_FUSED_F64 = 0, CC_PROF_LP = 1, CC_PROF_KINDS = 2 };
+int cc_prof_start(hipStream_t st, hipEvent_t* e0);
+int cc_prof_stop(hipStream_t st, int kind, hipEvent_t e0);

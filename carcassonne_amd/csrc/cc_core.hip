@@ -1,6 +1,8 @@
 // Data-protocol primitives: permute, elementwise, fixed-order reductions, casts.
 #include "cc_common.h"
 
+#include <vector>
+
 #include <cstring>
 #include <string>
 
@@ -176,6 +178,8 @@ extern "C" int cc_cast(void* stream, int64_t n, int src_dtype, const void* src, 
   else if (src_dtype == CC_F64 && dst_dtype == CC_BF16) cast_f64_bf16_kernel<<<b, 256, 0, st>>>(n, (const double*)src, (__bf16*)dst);
   else if (src_dtype == CC_F16 && dst_dtype == CC_F64) cast_to_f64_kernel<_Float16><<<b, 256, 0, st>>>(n, (const _Float16*)src, (double*)dst);
   else if (src_dtype == CC_BF16 && dst_dtype == CC_F64) cast_to_f64_kernel<__bf16><<<b, 256, 0, st>>>(n, (const __bf16*)src, (double*)dst);
+  else if (src_dtype == CC_F32 && dst_dtype == CC_F16) cast_kernel<float, _Float16><<<b, 256, 0, st>>>(n, (const float*)src, (_Float16*)dst);
+  else if (src_dtype == CC_F16 && dst_dtype == CC_F32) cast_kernel<_Float16, float><<<b, 256, 0, st>>>(n, (const _Float16*)src, (float*)dst);
   else { cc_set_last_error("cc_cast: unsupported dtype pair"); return CC_ERR_UNSUPPORTED; }
   CC_LAUNCH_CHECK();
   return CC_OK;
@@ -229,5 +233,64 @@ extern "C" int cc_scale_rows_cols_f64(void* stream, int cols, int64_t m, int64_t
   if (m <= 0 || n <= 0) return CC_OK;
   scale_rc_kernel<<<cc_blocks(m * n, 256), 256, 0, cc_stream(stream)>>>(cols, m, n, A, s, out);
   CC_LAUNCH_CHECK();
+  return CC_OK;
+}
+
+// ---------------------------------------------------------------- launch timing
+namespace {
+bool g_profile = false;
+struct ProfRecord {
+  int kind;
+  hipEvent_t e0, e1;
+};
+std::vector<ProfRecord> g_prof;
+void prof_clear() {
+  for (const ProfRecord& r : g_prof) {
+    (void)hipEventDestroy(r.e0);
+    (void)hipEventDestroy(r.e1);
+  }
+  g_prof.clear();
+}
+}  // namespace
+
+int cc_prof_start(hipStream_t st, hipEvent_t* e0) {
+  *e0 = nullptr;
+  if (!g_profile) return CC_OK;
+  CC_HIP(hipEventCreate(e0));
+  CC_HIP(hipEventRecord(*e0, st));
+  return CC_OK;
+}
+
+int cc_prof_stop(hipStream_t st, int kind, hipEvent_t e0) {
+  if (!g_profile || !e0) return CC_OK;
+  hipEvent_t e1 = nullptr;
+  CC_HIP(hipEventCreate(&e1));
+  CC_HIP(hipEventRecord(e1, st));
+  g_prof.push_back(ProfRecord{kind, e0, e1});
+  return CC_OK;
+}
+
+extern "C" int cc_profile_begin(void) {
+  prof_clear();
+  g_profile = true;
+  return CC_OK;
+}
+
+extern "C" int cc_profile_end(int n_kinds, double* total_ms, int64_t* launches) {
+  g_profile = false;
+  for (int k = 0; k < n_kinds; ++k) {
+    total_ms[k] = 0.0;
+    launches[k] = 0;
+  }
+  for (const ProfRecord& r : g_prof) {
+    CC_HIP(hipEventSynchronize(r.e1));
+    float ms = 0.0f;
+    CC_HIP(hipEventElapsedTime(&ms, r.e0, r.e1));
+    if (r.kind >= 0 && r.kind < n_kinds) {
+      total_ms[r.kind] += ms;
+      launches[r.kind] += 1;
+    }
+  }
+  prof_clear();
   return CC_OK;
 }

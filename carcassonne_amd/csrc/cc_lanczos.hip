@@ -375,12 +375,59 @@ __global__ void cg_axpby_kernel(int64_t n, const double* __restrict__ sa, double
 }
 
 // ratio[0] = num[0] / den[0] (0 when den == 0)
-__global__ void ratio_kernel(const double* num, const double* den, double* out) {
-  out[0] = den[0] != 0.0 ? num[0] / den[0] : 0.0;
-}
 
 // lowest eigenpair of the k x k pencil (Hs, Ns), k <= 32, single thread: Cholesky of Ns, cyclic Jacobi
 // on L^-1 Hs L^-T, back-transform.  out[0] = value, vec[0..k) N-normalised coefficients.
+// Sum of v over the 1024 threads of the workgroup in a fixed order (wave shuffles, then the 16 wave
+// sums in wave order); every thread gets the result.
+__device__ inline double cg_block_sum(double v, double* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int k = 0; k < 16; ++k) t += red[k];
+  return t;
+}
+
+// One CG step of the projected correction equation after ap = (W Hlp W^T) p, in one 1024-thread
+// workgroup:  ap -= rho p;  ap -= u (u . ap);  alpha = rr / (p . ap);  y += alpha p;  res -= alpha ap;
+// rr' = res . res;  p = res + (rr' / rr) p;  rr = rr'.   Fixed-order reductions.
+__global__ void __launch_bounds__(1024) cg_step_kernel(int64_t n, double rho, const double* __restrict__ u,
+                                                       double* __restrict__ ap, double* __restrict__ p,
+                                                       double* __restrict__ y, double* __restrict__ res,
+                                                       double* __restrict__ rr) {
+  __shared__ double red[16];
+  double du = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) {
+    const double a = ap[i] - rho * p[i];
+    ap[i] = a;
+    du += u[i] * a;
+  }
+  const double c = cg_block_sum(du, red);
+  double pap = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) {
+    const double a = ap[i] - c * u[i];
+    ap[i] = a;
+    pap += p[i] * a;
+  }
+  const double alpha = rr[0] / cg_block_sum(pap, red);
+  double rn = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) {
+    y[i] += alpha * p[i];
+    const double r = res[i] - alpha * ap[i];
+    res[i] = r;
+    rn += r * r;
+  }
+  const double rr_new = cg_block_sum(rn, red);
+  const double beta = rr_new / rr[0];
+  for (int64_t i = threadIdx.x; i < n; i += 1024) p[i] = res[i] + beta * p[i];
+  __syncthreads();
+  if (threadIdx.x == 0) rr[0] = rr_new;
+}
+
 __global__ void small_geneig_kernel(int k, const double* __restrict__ Hs, const double* __restrict__ Ns,
                                     double* __restrict__ out, double* __restrict__ vec) {
   if (threadIdx.x != 0) return;
@@ -524,24 +571,8 @@ extern "C" int cc_jd_correction_lp(void* stream, const void* lp_plan, int64_t n_
     // ap = P (W H W^T - rho) p   (p is kept in the projected space)
     rc = op_whitened(st, cc_internal_lp_apply, lp_plan, wh, p, t1, t2, ap);
     if (rc) return rc;
-    cg_axpby_kernel<<<gb, 256, 0, st>>>(n, nullptr, -rho, p, nullptr, 1.0, ap);
+    cg_step_kernel<<<1, 1024, 0, st>>>(n, rho, u, ap, p, y, res, sc + 1);
     CC_LAUNCH_CHECK();
-    if ((rc = project(ap))) return rc;
-    rc = cc_dot_f64(st, n, p, ap, sc + 3);
-    if (rc) return rc;
-    ratio_kernel<<<1, 1, 0, st>>>(sc + 1, sc + 3, sc + 4);                 // alpha = rr / pAp
-    CC_LAUNCH_CHECK();
-    cg_axpby_kernel<<<gb, 256, 0, st>>>(n, sc + 4, 1.0, p, nullptr, 1.0, y);  // y += alpha p
-    CC_LAUNCH_CHECK();
-    cg_axpby_kernel<<<gb, 256, 0, st>>>(n, sc + 4, -1.0, ap, nullptr, 1.0, res);  // res -= alpha ap
-    CC_LAUNCH_CHECK();
-    rc = cc_dot_f64(st, n, res, res, sc + 2);
-    if (rc) return rc;
-    ratio_kernel<<<1, 1, 0, st>>>(sc + 2, sc + 1, sc + 5);                 // beta = rr_new / rr
-    CC_LAUNCH_CHECK();
-    cg_axpby_kernel<<<gb, 256, 0, st>>>(n, nullptr, 1.0, res, sc + 5, 1.0, p);   // p = res + beta p
-    CC_LAUNCH_CHECK();
-    CC_HIP(hipMemcpyAsync(sc + 1, sc + 2, sizeof(double), hipMemcpyDeviceToDevice, st));
   }
   if ((rc = project(y))) return rc;
   return cc_skinny_f64(st, n_env, 2, Wt, (int)d, y, t_out);                // t = W^T P y

@@ -6,82 +6,120 @@
 namespace {
 constexpr int CB = 64;  // Cholesky block size
 
-__device__ inline double bcast(double v, int lane) {
-  // value of `v` in lane `lane` (a compile-time constant at every call site) through two readlanes
-  int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-  int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-  return __hiloint2double(hi, lo);
-}
-
-// Cholesky of the kb x kb diagonal block at (k0, k0) on one wave: lane i holds row i in registers,
-// columns are broadcast with readlane.  The factor is staged in LDS and inverted (lane c solves
-// column c of L x = e_c with L rows read as LDS broadcasts).  Writes L back to A and L^-1 to dinv.
-__global__ void __launch_bounds__(64) chol_diag_kernel(int64_t n, int64_t k0, int kb, double* __restrict__ A,
+// Cholesky of the kb x kb diagonal block at (k0, k0) in LDS on one 256-thread workgroup (kb <= CB; the
+// block is padded to CB with the identity).  Step k updates the trailing lower triangle with the
+// unscaled column k and the pivot (a[i][j] -= a[i][k] a[j][k] / a[k][k]), one barrier per step; the
+// columns are scaled at the end.  The factor is then inverted in place column by column from the last
+// (lower-triangular TRTI2 order: column j of L^-1 below the diagonal is -L^-1[j+1:, j+1:] L[j+1:, j] /
+// L[j][j]), four threads per row sharing each dot product.  Writes L back to A and L^-1 to dinv
+// (kb x kb row-major); info receives the first non-positive pivot (1-based, global index).
+__global__ void __launch_bounds__(256) chol_diag_kernel(int64_t n, int64_t k0, int kb, double* __restrict__ A,
                                                        double* __restrict__ dinv, int* __restrict__ info) {
-  const int i = threadIdx.x;
-  double r[CB];
-#pragma unroll
-  for (int c = 0; c < CB; ++c) r[c] = (i < kb && c <= i) ? A[(k0 + i) * n + k0 + c] : (i == c ? 1.0 : 0.0);
+  __shared__ double a[CB][CB + 1];
+  __shared__ double piv[CB];
+  __shared__ double col[CB];
+  const int t = threadIdx.x;
+  for (int e = t; e < CB * CB; e += 256) {
+    const int i = e / CB, j = e % CB;
+    a[i][j] = (i < kb && j <= i) ? A[(k0 + i) * n + k0 + j] : (i == j ? 1.0 : 0.0);
+  }
+  __syncthreads();
   int bad = 0;
-#pragma unroll
-  for (int j = 0; j < CB; ++j) {
-    double djj = bcast(r[j], j);
-    if (j < kb && !(djj > 0.0)) {
-      if (!bad) bad = j + 1;   // report the first non-positive pivot
-      djj = 1e-300;
+  const int row = t >> 2, part = t & 3;  // four threads per row of the trailing triangle
+  for (int k = 0; k < CB; ++k) {
+    double d = a[k][k];
+    if (!(d > 0.0)) {
+      if (k < kb && !bad) bad = k + 1;
+      d = 1e-300;
     }
-    const double sj = sqrt(djj);
-    r[j] = (i == j) ? sj : (i > j ? r[j] / sj : r[j]);
-#pragma unroll
-    for (int c = j + 1; c < CB; ++c) {
-      double acj = bcast(r[j], c);
-      if (c <= i) r[c] -= r[j] * acj;
+    if (t == 0) piv[k] = d;
+    if (row > k) {
+      const double l = a[row][k] / d;
+#pragma unroll 4
+      for (int j = k + 1 + part; j <= row; j += 4) a[row][j] -= l * a[j][k];
     }
+    __syncthreads();
   }
-  if (i == 0 && bad && *info == 0) *info = (int)(k0 + bad);
-  // inverse: column `i` of X = L^-1 (row `row` of L is lane `row`'s registers)
-  double x[CB];
-#pragma unroll
-  for (int row = 0; row < CB; ++row) {
-    double acc = (row == i) ? 1.0 : 0.0;
-#pragma unroll
-    for (int k = 0; k < row; ++k) acc -= bcast(r[k], row) * x[k];
-    x[row] = (row < i) ? 0.0 : acc / bcast(r[row], row);
+  if (t == 0 && bad && *info == 0) *info = (int)(k0 + bad);
+  // L[i][j] = a[i][j] / sqrt(piv[j]) below the diagonal, sqrt(piv[i]) on it
+  for (int e = t; e < CB * CB; e += 256) {
+    const int i = e / CB, j = e % CB;
+    if (j < i) a[i][j] *= 1.0 / sqrt(piv[j]);
+    else if (j == i) a[i][i] = sqrt(piv[i]);
   }
-  if (i < kb) {
-#pragma unroll
-    for (int c = 0; c < CB; ++c)
-      if (c < kb) A[(k0 + i) * n + k0 + c] = (c <= i) ? r[c] : 0.0;
+  __syncthreads();
+  for (int e = t; e < kb * kb; e += 256) {
+    const int i = e / kb, j = e % kb;
+    A[(k0 + i) * n + k0 + j] = (j <= i) ? a[i][j] : 0.0;
   }
-  // dinv[row][i] = x[row]
-#pragma unroll
-  for (int row = 0; row < CB; ++row)
-    if (row < kb && i < kb) dinv[row * kb + i] = x[row];
+  __syncthreads();
+  // in-place inverse, columns from the last; rows i > j: 4 lanes per row split the dot product
+  for (int j = CB - 1; j >= 0; --j) {
+    const double ljj = a[j][j];
+    double acc = 0.0;
+    if (row > j) {
+      double acc2 = 0.0;
+      int k = j + 1 + part;
+      for (; k + 4 <= row; k += 8) {  // two independent chains, loads issued ahead
+        acc += a[row][k] * a[k][j];
+        acc2 += a[row][k + 4] * a[k + 4][j];
+      }
+      if (k <= row) acc += a[row][k] * a[k][j];
+      acc += acc2;
+    }
+    acc += __shfl_xor(acc, 1, 64);
+    acc += __shfl_xor(acc, 2, 64);
+    if (part == 0 && row > j) col[row] = -acc / ljj;
+    __syncthreads();
+    if (part == 0 && row > j) a[row][j] = col[row];
+    if (t == 0) a[j][j] = 1.0 / ljj;
+    __syncthreads();
+  }
+  for (int e = t; e < kb * kb; e += 256) {
+    const int i = e / kb, j = e % kb;
+    dinv[i * kb + j] = (j <= i) ? a[i][j] : 0.0;
+  }
 }
 
-// Inverse of every diagonal block of L into X (block g on workgroup g).
+// Inverse of every diagonal block of L into X (block g on workgroup g): in-place TRTI2 order in LDS as in
+// chol_diag_kernel (columns from the last, four threads per row per dot product).
 __global__ void __launch_bounds__(256) diag_block_inverse_kernel(int64_t n, const double* __restrict__ L,
                                                                  double* __restrict__ X) {
   __shared__ double a[CB][CB + 1];
-  __shared__ double x[CB][CB + 1];
+  __shared__ double col[CB];
   const int64_t k0 = (int64_t)blockIdx.x * CB;
   const int kb = (int)((n - k0) < CB ? (n - k0) : CB);
-  for (int e = threadIdx.x; e < kb * kb; e += 256) {
-    int i = e / kb, j = e % kb;
-    a[i][j] = L[(k0 + i) * n + k0 + j];
+  const int t = threadIdx.x;
+  for (int e = t; e < CB * CB; e += 256) {
+    const int i = e / CB, j = e % CB;
+    a[i][j] = (i < kb && j <= i) ? L[(k0 + i) * n + k0 + j] : (i == j ? 1.0 : 0.0);
   }
   __syncthreads();
-  for (int c = threadIdx.x; c < kb; c += 256) {
-    for (int i = 0; i < kb; ++i) {
-      double v = (i == c) ? 1.0 : 0.0;
-      for (int k = c; k < i; ++k) v -= a[i][k] * x[k][c];
-      x[i][c] = (i < c) ? 0.0 : v / a[i][i];
+  const int row = t >> 2, part = t & 3;
+  for (int j = CB - 1; j >= 0; --j) {
+    const double ljj = a[j][j];
+    double acc = 0.0;
+    if (row > j) {
+      double acc2 = 0.0;
+      int k = j + 1 + part;
+      for (; k + 4 <= row; k += 8) {  // two independent chains, loads issued ahead
+        acc += a[row][k] * a[k][j];
+        acc2 += a[row][k + 4] * a[k + 4][j];
+      }
+      if (k <= row) acc += a[row][k] * a[k][j];
+      acc += acc2;
     }
+    acc += __shfl_xor(acc, 1, 64);
+    acc += __shfl_xor(acc, 2, 64);
+    if (part == 0 && row > j) col[row] = -acc / ljj;
+    __syncthreads();
+    if (part == 0 && row > j) a[row][j] = col[row];
+    if (t == 0) a[j][j] = 1.0 / ljj;
+    __syncthreads();
   }
-  __syncthreads();
-  for (int e = threadIdx.x; e < kb * kb; e += 256) {
-    int i = e / kb, j = e % kb;
-    X[(k0 + i) * n + k0 + j] = x[i][j];
+  for (int e = t; e < kb * kb; e += 256) {
+    const int i = e / kb, j = e % kb;
+    X[(k0 + i) * n + k0 + j] = (j <= i) ? a[i][j] : 0.0;
   }
 }
 
@@ -113,8 +151,15 @@ __global__ void __launch_bounds__(256) skinny_kernel(int64_t n, int tri, const d
 #pragma unroll
   for (int s = 0; s < NR; ++s) acc[s] = 0.0;
   const double* mr = M + row * n;
-  for (int64_t k = kb + lane; k < ke; k += 64) {
-    double m = mr[k];
+  int64_t k = kb + lane;
+  for (; k + 192 < ke; k += 256) {  // four independent row loads in flight per lane
+    const double m0 = mr[k], m1 = mr[k + 64], m2 = mr[k + 128], m3 = mr[k + 192];
+#pragma unroll
+    for (int s = 0; s < NR; ++s)
+      acc[s] += m0 * X[k * NR + s] + m1 * X[(k + 64) * NR + s] + m2 * X[(k + 128) * NR + s] + m3 * X[(k + 192) * NR + s];
+  }
+  for (; k < ke; k += 64) {
+    const double m = mr[k];
 #pragma unroll
     for (int s = 0; s < NR; ++s) acc[s] += m * X[k * NR + s];
   }
@@ -155,7 +200,7 @@ extern "C" int cc_cholesky_f64(void* stream, int64_t n, double* A, int* info_dev
   CC_LAUNCH_CHECK();
   for (int64_t k0 = 0; k0 < n; k0 += CB) {
     int kb = (int)((n - k0) < CB ? (n - k0) : CB);
-    chol_diag_kernel<<<1, 64, 0, st>>>(n, k0, kb, A, dinv, info_dev);
+    chol_diag_kernel<<<1, 256, 0, st>>>(n, k0, kb, A, dinv, info_dev);
     CC_LAUNCH_CHECK();
     int64_t rest = n - k0 - kb;
     if (rest > 0) {

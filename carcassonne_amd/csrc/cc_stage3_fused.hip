@@ -18,8 +18,6 @@
 
 namespace {
 // Optional timing of every fused-kernel launch (HIP events on the launch stream).
-bool g_profile = false;
-std::vector<hipEvent_t> g_prof_events;  // start/stop pairs
 
 constexpr int MAX_CH = 16, MAX_TERMS = 48, MAX_OPS = 16, MAX_D = 4;
 
@@ -186,19 +184,13 @@ __global__ void __launch_bounds__(256) reduce_partials_kernel(int64_t groups, in
 
 template <int P, int DP>
 int launch_fused(hipStream_t st, const FusedPlan& plan, const double* v, double* y, double* work, int groups) {
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (g_profile) {
-    CC_HIP(hipEventCreate(&e0));
-    CC_HIP(hipEventCreate(&e1));
-    CC_HIP(hipEventRecord(e0, st));
-  }
+  hipEvent_t e0 = nullptr;
+  int rc = cc_prof_start(st, &e0);
+  if (rc != CC_OK) return rc;
   fused_f64_kernel<P, DP><<<groups, 256, 0, st>>>(plan, v, work);
   CC_LAUNCH_CHECK();
-  if (g_profile) {
-    CC_HIP(hipEventRecord(e1, st));
-    g_prof_events.push_back(e0);
-    g_prof_events.push_back(e1);
-  }
+  rc = cc_prof_stop(st, CC_PROF_FUSED_F64, e0);
+  if (rc != CC_OK) return rc;
   const int64_t n = (int64_t)P * P * DP;
   reduce_partials_kernel<<<(unsigned)((n + 15) / 16), 256, 0, st>>>(groups, n, work, y);
   CC_LAUNCH_CHECK();
@@ -345,25 +337,3 @@ extern "C" int cc_stage3_plan_destroy(void* plan) {
   return CC_OK;
 }
 
-extern "C" int cc_profile_begin(void) {
-  for (hipEvent_t e : g_prof_events) (void)hipEventDestroy(e);
-  g_prof_events.clear();
-  g_profile = true;
-  return CC_OK;
-}
-
-extern "C" int cc_profile_end(double* total_ms, int64_t* launches) {
-  g_profile = false;
-  double total = 0.0;
-  for (size_t i = 0; i + 1 < g_prof_events.size(); i += 2) {
-    CC_HIP(hipEventSynchronize(g_prof_events[i + 1]));
-    float ms = 0.0f;
-    CC_HIP(hipEventElapsedTime(&ms, g_prof_events[i], g_prof_events[i + 1]));
-    total += ms;
-  }
-  *total_ms = total;
-  *launches = (int64_t)(g_prof_events.size() / 2);
-  for (hipEvent_t e : g_prof_events) (void)hipEventDestroy(e);
-  g_prof_events.clear();
-  return CC_OK;
-}

@@ -443,9 +443,14 @@ int launch_lp(hipStream_t st, const LPHandle* h, const double* v, double* y) {
   lp_prepare_kernel<P, DP, T><<<1, 1024, 0, st>>>(v, fhi, flo, scale);
   CC_LAUNCH_CHECK();
   const dim3 grid((unsigned)h->groups, (unsigned)h->chunks);
+  hipEvent_t e0 = nullptr;
+  int rc = cc_prof_start(st, &e0);
+  if (rc != CC_OK) return rc;
   if (h->split) lp_kernel<P, DP, T, true><<<grid, 64 * NT, 0, st>>>(plan, fhi, flo, partial);
   else lp_kernel<P, DP, T, false><<<grid, 64 * NT, 0, st>>>(plan, fhi, flo, partial);
   CC_LAUNCH_CHECK();
+  rc = cc_prof_stop(st, CC_PROF_LP, e0);
+  if (rc != CC_OK) return rc;
   lp_reduce_kernel<<<(unsigned)((n + LP_RCOLS - 1) / LP_RCOLS), LP_RCOLS * LP_RGROUPS, 0, st>>>(
       (int64_t)h->groups * h->chunks, n, partial, scale, y);
   CC_LAUNCH_CHECK();

@@ -53,7 +53,7 @@ class LowPrecisionOperator:
         for (a, _, b, O) in rows:
             m = np.identity(self.d) if O is None else O.toArray()
             coef.extend((sa[a] * sb[b] * m).ravel().tolist())
-        self.groups = int(groups if groups is not None else max(1, min(self.n_alpha // 4, 2048)))
+        self.groups = int(groups if groups is not None else max(1, min(self.n_alpha // 8, 2048)))
         self._work = torch.empty(_lib.query("cc_stage3_lp_work_size", self.n_alpha, self.P, self.d, self.groups),
                                  dtype=torch.float32, device=sc.device)
         self._ptrs_a = _lib.ptr_array([t.data_ptr() for t in self._slabs[:len(a_list)]])
@@ -93,7 +93,11 @@ class MixedSolver:
         self._lz_work, self._lz_max = None, 0
         self._jd_work = empty((_lib.query("cc_jd_work_size", self.n),))
 
-    def solve(self, x0, max_lowprec_iter=300, check_every=4, inner_iters=20, max_outer=8, tol_energy=1e-12):
+    def solve(self, x0, max_lowprec_iter=300, check_every=4, inner_iters=20, max_outer=8, tol_energy=1e-13,
+              tol_predicted=1e-11, lp_tol_residual=0.0, lp_tol_energy=1e-9):
+        """Stops after the FP64 Rayleigh-Ritz value rho_k when |rho_k - rho_{k-1}| <= tol_energy |rho_k|, or
+        when the remaining error predicted from the last two changes (geometric convergence,
+        d_k * q / (1 - q) with q = d_k / d_{k-1}) is at most tol_predicted |rho_k|."""
         from .data import HipData
         if isinstance(x0, HipData):
             x0 = x0._t.reshape(-1)
@@ -106,7 +110,8 @@ class MixedSolver:
         res = _lib.f64_array([0, 0, 0])
         # (1) low-precision Lanczos until the Ritz value stops moving at the precision floor
         _lib.call("cc_lanczos_lp", stream(), self.Hlp.plan, f.n_env, f.d, ptr(f.L), ptr(f.Lt), ptr(f.W), ptr(f.Wt),
-                  ptr(x0), 0.0, 1e-9, max_lowprec_iter, int(check_every), ptr(self._lz_work), ptr(x), res)
+                  ptr(x0), float(lp_tol_residual), float(lp_tol_energy), max_lowprec_iter, int(check_every),
+                  ptr(self._lz_work), ptr(x), res)
         lp_iters = int(res[2])
         # (2) Jacobi-Davidson in FP64
         kmax = max_outer + 1
@@ -116,6 +121,7 @@ class MixedSolver:
         U[0].copy_(x)
         k, rho, prev = 0, None, None
         fp64_passes = 0
+        history = []
         for outer in range(max_outer):
             # FP64 H and N on the newest basis vector, then the projected pencil
             _lib.call("cc_stage3_plan_apply", stream(), self._hplan, ptr(U[k]), ptr(HU[k]))
@@ -128,10 +134,17 @@ class MixedSolver:
                 _lib.call("cc_basis_dots_f64", stream(), k, n, ptr(U), ptr(NU[j]), ptr(ns[:, j]), k)
             _lib.call("cc_small_geneig_f64", stream(), k, ptr(hs), ptr(ns), ptr(small[0:1]), ptr(small[1:1 + k]))
             rho = float(small[0].item())
+            history.append(rho)
             coeffs = small[1:1 + k].contiguous()
             _lib.call("cc_basis_combine_f64", stream(), k, n, ptr(U), ptr(coeffs), ptr(xr))
-            if prev is not None and abs(prev - rho) <= tol_energy * abs(rho):
-                break
+            if prev is not None:
+                delta = abs(prev - rho)
+                if delta <= tol_energy * abs(rho):
+                    break
+                if len(history) >= 3:
+                    q = delta / max(abs(history[-2] - history[-3]), 1e-300)
+                    if q < 1.0 and delta * q / (1.0 - q) <= tol_predicted * abs(rho):
+                        break
             prev = rho
             if outer == max_outer - 1:
                 break
@@ -152,4 +165,4 @@ class MixedSolver:
             _lib.call("cc_dot_f64", stream(), n, ptr(t), ptr(t), ptr(nrm))
             _lib.call("cc_axpby_f64", stream(), n, 1.0 / np.sqrt(float(nrm.item())), ptr(t), 0.0, ptr(U[k]))
         return xr, rho, dict(lowprec_iterations=lp_iters, outer_steps=k, fp64_passes=fp64_passes,
-                             lowprec_ritz=float(res[0]))
+                             lowprec_ritz=float(res[0]), lowprec_residual=float(res[1]), history=history)

@@ -67,9 +67,26 @@ def pair_is_antisymmetric(model):
     return [False] if model == "ising" else [False, True, False]
 
 
-def make_environment(D, chi, d=2, model="ising", seed=0, energy=-1.0, local=0.3, noise=1.0):
-    """Returns (corners, sides, state) -- lists of 4 dicts and the state array."""
-    rng = np.random.Generator(np.random.PCG64(seed))
+class _Mixed:
+    """standard_normal draws of a base generator plus sigma times the same draws of a second one: the
+    environments of one family share the base environment and differ by a seeded perturbation."""
+
+    def __init__(self, seed, perturb_seed, sigma):
+        self.a = np.random.Generator(np.random.PCG64(seed))
+        self.b = np.random.Generator(np.random.PCG64(perturb_seed)) if perturb_seed is not None else None
+        self.sigma = sigma
+
+    def standard_normal(self, shape):
+        x = self.a.standard_normal(shape)
+        return x if self.b is None else x + self.sigma * self.b.standard_normal(shape)
+
+
+def make_environment(D, chi, d=2, model="ising", seed=0, energy=-1.0, local=0.3, noise=1.0, perturb_seed=None,
+                     sigma=0.0):
+    """Returns (corners, sides, state) -- lists of 4 dicts and the state array.  With perturb_seed every
+    raw draw of the seed-`seed` environment gets sigma times the same draw of generator `perturb_seed`
+    (before the positivity and symmetry constructions, which therefore still hold)."""
+    rng = _Mixed(seed, perturb_seed, sigma)
     npairs = n_pairs(model)
 
     def sym_noise(shape, sign=1.0):

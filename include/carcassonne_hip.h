@@ -135,11 +135,12 @@ int cc_stage3_plan_create(void** out, int n_a, const double* const* A_packed, in
 int cc_stage3_plan_apply(void* stream, const void* plan, const double* x, double* y);
 int cc_stage3_plan_destroy(void* plan);
 
-/* Timing of the fused stage-3 kernel: between begin and end every launch is bracketed by
- * HIP events on its stream; end returns the summed kernel time and the launch count.
- * Single-threaded use only.  (Measurement support; no reference counterpart.) */
+/* Timing of the stage-3 kernels: between begin and end every launch of the FP64 fused kernel (kind 0)
+ * and of the low-precision kernel (kind 1) is bracketed by HIP events on its stream; end fills
+ * total_ms[k] and launches[k] for k < n_kinds.  Single-threaded use only.  (Measurement support; no
+ * reference counterpart.) */
 int cc_profile_begin(void);
-int cc_profile_end(double* total_ms, int64_t* launches);
+int cc_profile_end(int n_kinds, double* total_ms, int64_t* launches);
 
 /* ---- dense linear algebra of the solve --------------------------------------------
  * In-place blocked Cholesky A = L L^T (lower; upper zeroed); info_dev[0] = 0 or the first
@@ -206,6 +207,30 @@ int cc_jd_work_size(int64_t n, int64_t* out);
 int cc_jd_correction_lp(void* stream, const void* lp_plan, int64_t n_env, int64_t d, const double* Lt,
                         const double* W, const double* Wt, double rho, const double* x, const double* r,
                         int iters, double* work, double* t_out);
+
+/* ---- learned warm start: a 3-layer MLP proposing the eigensolver's start vector ----------------
+ * (no reference counterpart: the reference always starts relaxOver from the current center,
+ * system/_2d.py:491-492; the plain solve stays the judge of every proposal)
+ * Network x (K0) -> relu(W1 x + b1) (H) -> relu(W2 . + b2) (H) -> W3 . + b3 (NO); parameters in one FP32
+ * buffer [W1 (H x K0), b1, W2 (H x H), b2, W3 (NO x H), b3] (cc_mlp_param_count floats) with an FP16
+ * mirror P16 of the same layout.  K0 % 16 == 0; H, NO multiples of 64; X16 is B x K0 FP16.
+ * cc_mlp_features: one FP16 feature row from n_spec spectra of k singular values (each divided by its
+ *   largest value) followed by h, zero-padded to K0.
+ * cc_mlp_forward: FP16 operands, FP32 accumulation on matrix cores; A16 (2 B H halves) and A32 (2 B H
+ *   floats) receive the hidden layers, Y (B x NO floats) the output.
+ * cc_mlp_backward: loss[b] = 1 - (y_b . v_b)^2 / (|y_b|^2 |v_b|^2) against target v (B x NO floats) and the
+ *   gradient of the batch mean into G (parameter layout); work holds cc_mlp_work_size floats.
+ * cc_mlp_adam: one Adam step on P32 with moments M, V (step counts from 1); refreshes P16. */
+int cc_mlp_param_count(int64_t K0, int64_t H, int64_t NO, int64_t* out);
+int cc_mlp_work_size(int64_t B, int64_t H, int64_t NO, int64_t* out);
+int cc_mlp_features(void* stream, int n_spec, int k, const double* S, double h, int64_t K0, void* out16);
+int cc_mlp_forward(void* stream, int64_t B, int64_t K0, int64_t H, int64_t NO, const void* P16, const float* P32,
+                   const void* X16, void* A16, float* A32, float* Y);
+int cc_mlp_backward(void* stream, int64_t B, int64_t K0, int64_t H, int64_t NO, const void* P16, const void* X16,
+                    const void* A16, const float* A32, const float* Y, const float* target, float* loss, float* G,
+                    float* work);
+int cc_mlp_adam(void* stream, int64_t n, float* P32, const float* G, float* M, float* V, void* P16, double lr,
+                double beta1, double beta2, double eps, int64_t step);
 
 #ifdef __cplusplus
 }
