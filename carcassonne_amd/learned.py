@@ -179,7 +179,7 @@ def solve_from(corners, sides, op_center, x0, check_every=1, tol_residual=1e-10)
     return r["x"], r["energy"], r["iterations"]
 
 
-def generate_pairs(family, model_net, sample_seeds, h_values):
+def generate_pairs(family, model_net, sample_seeds, h_values, check_every=4):
     """Features (n, K0) FP16 and converged states (n, NO) FP32 (zero-padded to the network's output size)
     from plain solves started at the family's seeded random state."""
     n = len(sample_seeds)
@@ -189,7 +189,7 @@ def generate_pairs(family, model_net, sample_seeds, h_values):
     iters = []
     for i, (sd, h) in enumerate(zip(sample_seeds, h_values)):
         corners, sides, opc, x0 = family.environment(sd, h)
-        x, _, it = solve_from(corners, sides, opc, x0)
+        x, _, it = solve_from(corners, sides, opc, x0, check_every=check_every)
         model_net.features(corners, sides, h, out=X16[i])
         _lib.call("cc_cast", stream(), family.n, _lib.CC_F64, ptr(x), _lib.CC_F32, ptr(T[i]))
         iters.append(it)
@@ -200,21 +200,49 @@ def evaluate_starts(family, model_net, sample_seeds, h_values, seed=0):
     """Lanczos iterations to a 1e-10 residual on each held-out environment from three starts: a seeded
     random vector, the previous environment's converged state (the sequence is walked in order, as
     consecutive sweep steps), and the network's proposal.  Energies of the three solves agree."""
+    import time
     rng = np.random.Generator(np.random.PCG64(seed))
-    out = dict(random=[], previous=[], learned=[], energy_spread=[])
+    out = dict(random=[], previous=[], learned=[], energy_spread=[], ms_random=[], ms_previous=[], ms_learned=[],
+               overlap_learned=[], overlap_previous=[])
+
+    def overlap(a, b):
+        """|a . b|^2 / (|a|^2 |b|^2) on the device (cc_dot_f64)."""
+        d = empty((3,))
+        _lib.call("cc_dot_f64", stream(), family.n, ptr(a), ptr(b), ptr(d[0:1]))
+        _lib.call("cc_dot_f64", stream(), family.n, ptr(a), ptr(a), ptr(d[1:2]))
+        _lib.call("cc_dot_f64", stream(), family.n, ptr(b), ptr(b), ptr(d[2:3]))
+        ab, aa, bb = d.tolist()
+        return ab * ab / (aa * bb)
     prev = None
+
+    def timed(fn):
+        synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        synchronize()
+        return r, (time.perf_counter() - t0) * 1e3
+
     for sd, h in zip(sample_seeds, h_values):
         corners, sides, opc, _ = family.environment(sd, h)
         xr = torch.from_numpy(rng.standard_normal(family.n)).to(model_net.P32.device)
-        x1, e1, it_r = solve_from(corners, sides, opc, xr)
-        xl = model_net.propose(model_net.features(corners, sides, h))[:family.n].contiguous()
-        _, e2, it_l = solve_from(corners, sides, opc, xl)
+        (x1, e1, it_r), ms = timed(lambda: solve_from(corners, sides, opc, xr))
         out["random"].append(it_r)
+        out["ms_random"].append(ms)
+
+        def learned():
+            xl = model_net.propose(model_net.features(corners, sides, h))[:family.n].contiguous()
+            return solve_from(corners, sides, opc, xl)
+        (_, e2, it_l), ms = timed(learned)
         out["learned"].append(it_l)
+        out["ms_learned"].append(ms)
+        xl = model_net.propose(model_net.features(corners, sides, h))[:family.n].contiguous()
+        out["overlap_learned"].append(overlap(xl, x1))
         es = [e1, e2]
         if prev is not None:
-            _, e3, it_p = solve_from(corners, sides, opc, prev)
+            (_, e3, it_p), ms = timed(lambda: solve_from(corners, sides, opc, prev))
             out["previous"].append(it_p)
+            out["ms_previous"].append(ms)
+            out["overlap_previous"].append(overlap(prev, x1))
             es.append(e3)
         out["energy_spread"].append(max(es) - min(es))
         prev = x1
