@@ -67,7 +67,9 @@ SIGNATURES = {
     "cc_basis_combine_f64": [_vp, _i32, _i64, _vp, _vp, _vp],
     "cc_small_geneig_f64": [_vp, _i32, _vp, _vp, _vp, _vp],
     "cc_jd_work_size": [_i64, _pi64],
-    "cc_jd_correction_lp": [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _f64, _vp, _vp, _i32, _vp, _vp],
+    "cc_jd_correction_lp": [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp],
+    "cc_normalize_f64": [_vp, _i64, _vp, _vp],
+    "cc_axpby_dev_f64": [_vp, _i64, _vp, _f64, _vp, _f64, _vp],
     "cc_mlp_param_count": [_i64, _i64, _i64, _pi64],
     "cc_mlp_work_size": [_i64, _i64, _i64, _pi64],
     "cc_mlp_features": [_vp, _i32, _i32, _vp, _f64, _i64, _vp],

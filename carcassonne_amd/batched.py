@@ -1,0 +1,152 @@
+"""The batched form: many independent systems of uneven (D, chi) and fields solved by one HIP-graph launch.
+
+No reference counterpart (the reference solves one system at a time).  Each system gets its stage-1/2
+operators, its normalization factor and its low-precision plan eagerly (`prepare`); the eigensolves of
+all systems -- fixed-work low-precision Lanczos plus FP64 Rayleigh-Ritz refinement, no host reads
+(MixedSolver.solve_fixed) -- are captured once into one HIP graph, each system on its own stream branch
+forked from the capture stream, so one `replay` launches every solve and the GPU overlaps the small
+systems' kernels.  An autotuner picks, per shape bucket, the low-precision kernel's workgroup count by
+timing candidates on the bucket's first system, and the fixed work (Lanczos steps, refinement passes)
+from one adaptive solve per bucket with a safety margin.  The plain FP64 solve of every system is the
+judge of the batched energies.
+"""
+import time
+
+import numpy as np
+import torch
+
+from . import pipeline, synthetic as syn
+from .device import empty, synchronize
+from .precision import LowPrecisionOperator, MixedSolver
+from .tensors._2d import sparse as S2
+
+SHAPES = ((3, 24), (4, 32), (5, 48), (6, 64))
+
+
+def draw_specs(n, seed=0, shapes=SHAPES, h_range=(2.0, 4.0)):
+    """(D, chi, h, env_seed) of n systems: shapes uniform over `shapes`, h uniform, seeded."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    idx = rng.integers(0, len(shapes), n)
+    hs = rng.uniform(h_range[0], h_range[1], n)
+    return [(shapes[i][0], shapes[i][1], float(h), 1000 + k) for k, (i, h) in enumerate(zip(idx, hs))]
+
+
+def _time_lp(op, x, y, reps=20):
+    for _ in range(3):
+        op.apply_flat(x, y)
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        op.apply_flat(x, y)
+    synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def autotune_groups(H_operator, dtype, candidates=None):
+    """Workgroup count of the low-precision stage-3 kernel for this operator's shape: the fastest of the
+    candidates (alpha per workgroup 2..32) by measured matvec time."""
+    na = H_operator.n_alpha
+    if candidates is None:
+        candidates = sorted({max(1, na // k) for k in (2, 4, 8, 16, 32)})
+    x = empty((H_operator.n,)).fill_(1.0)
+    y = empty((H_operator.n,))
+    best, best_t = None, None
+    timings = {}
+    for g in candidates:
+        op = LowPrecisionOperator(H_operator, dtype, groups=g)
+        t = _time_lp(op, x, y)
+        timings[g] = t
+        if best_t is None or t < best_t:
+            best, best_t = g, t
+    return best, timings
+
+
+class BatchedSolve:
+    def __init__(self, specs, dtype="f16", model="ising"):
+        self.specs, self.dtype, self.model = list(specs), dtype, model
+        self.systems = []
+        self.tuning = {}
+        self.graph = None
+
+    def load(self):
+        """Generate every environment on the host and upload it (the input data; not part of a solve)."""
+        self.inputs = []
+        for (D, chi, h, seed) in self.specs:
+            c, s, st = syn.make_environment(D, chi, model=self.model, seed=seed)
+            corners, sides, opc = pipeline.device_environment(c, s, syn.operator_center(self.model, h=h))
+            x0 = torch.from_numpy(st.ravel()).to(empty((1,)).device)
+            self.inputs.append((corners, sides, opc, x0))
+        synchronize()
+
+    def autotune(self):
+        """Workgroup count of the low-precision kernel per shape bucket (on the bucket's first system)."""
+        for (corners, sides, opc, _) in self.inputs:
+            H, _ = S2.formExpectationAndNormalizationMultipliers(corners, sides, opc)
+            key = (H.operator.P, H.operator.n_alpha)
+            if key not in self.tuning:
+                g, timings = autotune_groups(H.operator, self.dtype)
+                self.tuning[key] = dict(groups=g, timings_us={k: v * 1e6 for k, v in timings.items()})
+        synchronize()
+
+    def build(self):
+        """Per system: stage 1/2 operators, normalization factor, low-precision plan (the solve's setup)."""
+        self.systems = []
+        for spec, (corners, sides, opc, x0) in zip(self.specs, self.inputs):
+            H, N = S2.formExpectationAndNormalizationMultipliers(corners, sides, opc)
+            key = (H.operator.P, H.operator.n_alpha)
+            solver = MixedSolver(H.operator, N.operator, dtype=self.dtype, groups=self.tuning[key]["groups"])
+            self.systems.append(dict(spec=spec, H=H, N=N, solver=solver, x0=x0, key=key))
+        synchronize()
+
+    def prepare(self):
+        self.load()
+        self.autotune()
+        self.build()
+
+    def tune_work(self, margin_iters=8, margin_outer=2):
+        """Fixed work per shape bucket from one adaptive solve of the bucket's first system."""
+        for key in self.tuning:
+            sysm = next(s for s in self.systems if s["key"] == key)
+            _, _, info = sysm["solver"].solve(sysm["x0"])
+            self.tuning[key]["lp_iters"] = int(info["lowprec_iterations"] * 1.25) + margin_iters
+            self.tuning[key]["outer"] = int(info["fp64_passes"]) + margin_outer
+        synchronize()
+
+    def _solve_all(self, streams):
+        cur = torch.cuda.current_stream()
+        outs = []
+        for sysm, st in zip(self.systems, streams):
+            st.wait_stream(cur)
+            with torch.cuda.stream(st):
+                t = self.tuning[sysm["key"]]
+                outs.append(sysm["solver"].solve_fixed(sysm["x0"], lp_iters=t["lp_iters"], outer=t["outer"]))
+        for st in streams:
+            cur.wait_stream(st)
+        return outs
+
+    def capture(self):
+        """Capture every system's eigensolve into one HIP graph (each system on its own stream branch)."""
+        self._streams = [torch.cuda.Stream() for _ in self.systems]
+        self._solve_all(self._streams)            # warm-up outside the graph: first-use allocations
+        synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.outputs = self._solve_all(self._streams)
+        synchronize()
+
+    def replay(self):
+        self.graph.replay()
+
+    def energies(self):
+        synchronize()
+        return [float(rho.item()) for _, rho in self.outputs]
+
+    def reference_energies(self):
+        """Plain FP64 solve of every system (the judge)."""
+        out = []
+        for sysm in self.systems:
+            from .solver import GroundStateSolver
+            gs = GroundStateSolver(sysm["H"].operator, sysm["N"].operator, factor=sysm["solver"].factor)
+            x, _, _, _ = gs.solve(sysm["x0"])
+            out.append(gs.rayleigh_quotient(x))
+        return out

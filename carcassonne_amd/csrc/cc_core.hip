@@ -86,6 +86,19 @@ extern "C" int cc_axpby_f64(void* stream, int64_t n, double alpha, const double*
   CC_LAUNCH_CHECK();
   return CC_OK;
 }
+__global__ void axpby_dev_kernel(int64_t n, const double* __restrict__ a_dev, double a_scale, const double* __restrict__ x,
+                                 double beta, double* __restrict__ y) {
+  const double a = a_scale * a_dev[0];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = a * x[i] + beta * y[i];
+}
+extern "C" int cc_axpby_dev_f64(void* stream, int64_t n, const double* alpha_dev, double alpha_scale, const double* x,
+                                double beta, double* y) {
+  if (n <= 0) return CC_OK;
+  axpby_dev_kernel<<<cc_blocks(n, 256), 256, 0, cc_stream(stream)>>>(n, alpha_dev, alpha_scale, x, beta, y);
+  CC_LAUNCH_CHECK();
+  return CC_OK;
+}
 extern "C" int cc_fill_f64(void* stream, int64_t n, double value, double* y) {
   if (n <= 0) return CC_OK;
   fill_kernel<<<cc_blocks(n, 256), 256, 0, cc_stream(stream)>>>(n, value, y);

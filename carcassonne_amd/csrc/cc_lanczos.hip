@@ -207,6 +207,10 @@ __global__ void __launch_bounds__(64) lz_tridiag_kernel(int m, const double* __r
   out[1] = fabs(b[m - 1] * sv[m - 1]);
 }
 
+__global__ void set_nan_kernel(double* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = NAN;
+}
+
 struct Whitening {
   int64_t n_env, d;
   const double *L, *Lt, *W, *Wt;
@@ -238,7 +242,7 @@ extern "C" int cc_lanczos_f64(void* stream, const void* plan, int64_t n_env, int
                               const double* W, const double* Wt, const double* x0, double tol_residual,
                               double tol_energy, int max_iter, int check_every, double* work, double* x_out,
                               double* host_result /* theta, residual, iterations */) {
-  if (!plan || n_env * d <= 0 || max_iter < 1 || check_every < 1) { cc_set_last_error("lanczos: bad arguments"); return CC_ERR_ARG; }
+  if (!plan || n_env * d <= 0 || max_iter < 1 || check_every < 0) { cc_set_last_error("lanczos: bad arguments"); return CC_ERR_ARG; }
   if (cc_internal_stage3_n(plan) != n_env * d) { cc_set_last_error("lanczos: operator size does not match the whitening"); return CC_ERR_DIM; }
   return lanczos_driver(cc_stream(stream), cc_internal_stage3_apply, plan, n_env, d, L, Lt, W, Wt, x0, tol_residual,
                         tol_energy, max_iter, check_every, work, x_out, host_result);
@@ -247,7 +251,7 @@ extern "C" int cc_lanczos_f64(void* stream, const void* plan, int64_t n_env, int
 extern "C" int cc_lanczos_lp(void* stream, const void* lp_plan, int64_t n_env, int64_t d, const double* L, const double* Lt,
                              const double* W, const double* Wt, const double* x0, double tol_residual, double tol_energy,
                              int max_iter, int check_every, double* work, double* x_out, double* host_result) {
-  if (!lp_plan || n_env * d <= 0 || max_iter < 1 || check_every < 1) { cc_set_last_error("lanczos: bad arguments"); return CC_ERR_ARG; }
+  if (!lp_plan || n_env * d <= 0 || max_iter < 1 || check_every < 0) { cc_set_last_error("lanczos: bad arguments"); return CC_ERR_ARG; }
   if (cc_internal_lp_n(lp_plan) != n_env * d) { cc_set_last_error("lanczos: operator size does not match the whitening"); return CC_ERR_DIM; }
   return lanczos_driver(cc_stream(stream), cc_internal_lp_apply, lp_plan, n_env, d, L, Lt, W, Wt, x0, tol_residual,
                         tol_energy, max_iter, check_every, work, x_out, host_result);
@@ -278,10 +282,9 @@ static int lanczos_driver(hipStream_t st, ApplyFn apply, const void* plan, int64
   lz_normalize_kernel<<<1, 1024, 0, st>>>(n, w, Q, nullptr);
   CC_LAUNCH_CHECK();
 
-  {
-    const double nan2[2] = {NAN, NAN};
-    CC_HIP(hipMemcpyAsync(res, nan2, sizeof(nan2), hipMemcpyHostToDevice, st));
-  }
+  const bool fixed = check_every == 0;  // exactly max_iter steps, no host reads (capturable in a graph)
+  set_nan_kernel<<<1, 64, 0, st>>>(res, 2);
+  CC_LAUNCH_CHECK();
   double prev_theta = NAN, theta = NAN, resid = NAN;
   int m = 0;
   for (int j = 0; j < max_iter; ++j) {
@@ -301,7 +304,8 @@ static int lanczos_driver(hipStream_t st, ApplyFn apply, const void* plan, int64
     lz_normalize_kernel<<<1, 1024, 0, st>>>(n, w, Q + (int64_t)(j + 1) * n, beta + j);
     CC_LAUNCH_CHECK();
     m = j + 1;
-    if (m % check_every == 0 || m == max_iter || m == n) {
+    if (m == n) break;
+    if (!fixed && (m % check_every == 0 || m == max_iter)) {
       // Ritz value every check; the residual estimate (needs the Ritz vector) once the value has settled
       bool settle_check = false;
       for (int pass = 0; pass < 2; ++pass) {
@@ -329,7 +333,7 @@ static int lanczos_driver(hipStream_t st, ApplyFn apply, const void* plan, int64
   // the final Ritz vector of T_m
   lz_tridiag_kernel<<<1, 64, 0, st>>>(m, alpha, beta, res, s, scratch, 1);
   CC_LAUNCH_CHECK();
-  {
+  if (!fixed) {
     double h[2];
     CC_HIP(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, st));
     CC_HIP(hipStreamSynchronize(st));
@@ -341,10 +345,12 @@ static int lanczos_driver(hipStream_t st, ApplyFn apply, const void* plan, int64
   CC_LAUNCH_CHECK();
   rc = cc_skinny_f64(st, n_env, 2, Wt, (int)d, t2, x_out);
   if (rc) return rc;
-  CC_HIP(hipStreamSynchronize(st));
-  host_result[0] = theta;
-  host_result[1] = resid;
-  host_result[2] = (double)m;
+  if (!fixed) CC_HIP(hipStreamSynchronize(st));
+  if (host_result) {
+    host_result[0] = theta;   // NaN in the fixed mode: the Ritz value and residual stay on the device
+    host_result[1] = resid;
+    host_result[2] = (double)m;
+  }
   return CC_OK;
 }
 
@@ -395,11 +401,12 @@ __device__ inline double cg_block_sum(double v, double* red) {
 // One CG step of the projected correction equation after ap = (W Hlp W^T) p, in one 1024-thread
 // workgroup:  ap -= rho p;  ap -= u (u . ap);  alpha = rr / (p . ap);  y += alpha p;  res -= alpha ap;
 // rr' = res . res;  p = res + (rr' / rr) p;  rr = rr'.   Fixed-order reductions.
-__global__ void __launch_bounds__(1024) cg_step_kernel(int64_t n, double rho, const double* __restrict__ u,
+__global__ void __launch_bounds__(1024) cg_step_kernel(int64_t n, const double* __restrict__ rho_dev, const double* __restrict__ u,
                                                        double* __restrict__ ap, double* __restrict__ p,
                                                        double* __restrict__ y, double* __restrict__ res,
                                                        double* __restrict__ rr) {
   __shared__ double red[16];
+  const double rho = rho_dev[0];
   double du = 0.0;
   for (int64_t i = threadIdx.x; i < n; i += 1024) {
     const double a = ap[i] - rho * p[i];
@@ -430,8 +437,10 @@ __global__ void __launch_bounds__(1024) cg_step_kernel(int64_t n, double rho, co
 
 __global__ void small_geneig_kernel(int k, const double* __restrict__ Hs, const double* __restrict__ Ns,
                                     double* __restrict__ out, double* __restrict__ vec) {
+  // the four k x k work matrices live in LDS (no private scratch: concurrent launches on several
+  // streams would each need a large scratch allocation)
+  __shared__ double Lm[32][32], C[32][32], V[32][32], Li[32][32];
   if (threadIdx.x != 0) return;
-  double Lm[32][32], C[32][32], V[32][32], Li[32][32];
   for (int i = 0; i < k; ++i)
     for (int j = 0; j < k; ++j) { Lm[i][j] = 0.0; Li[i][j] = 0.0; V[i][j] = (i == j) ? 1.0 : 0.0; }
   for (int j = 0; j < k; ++j) {
@@ -526,8 +535,8 @@ extern "C" int cc_jd_work_size(int64_t n, int64_t* out) {
 //   (I - u u^T)(W Hlp W^T - rho)(I - u u^T) y = -(I - u u^T) W r,  u = L^T x / |L^T x|,  t = W^T (I - u u^T) y
 // by `iters` CG steps on the low-precision operator (all scalars on the device).
 extern "C" int cc_jd_correction_lp(void* stream, const void* lp_plan, int64_t n_env, int64_t d, const double* Lt,
-                                   const double* W, const double* Wt, double rho, const double* x, const double* r,
-                                   int iters, double* work, double* t_out) {
+                                   const double* W, const double* Wt, const double* rho_dev, const double* x,
+                                   const double* r, int iters, double* work, double* t_out) {
   hipStream_t st = cc_stream(stream);
   const int64_t n = n_env * d;
   if (cc_internal_lp_n(lp_plan) != n) { cc_set_last_error("jd: operator size mismatch"); return CC_ERR_DIM; }
@@ -571,9 +580,16 @@ extern "C" int cc_jd_correction_lp(void* stream, const void* lp_plan, int64_t n_
     // ap = P (W H W^T - rho) p   (p is kept in the projected space)
     rc = op_whitened(st, cc_internal_lp_apply, lp_plan, wh, p, t1, t2, ap);
     if (rc) return rc;
-    cg_step_kernel<<<1, 1024, 0, st>>>(n, rho, u, ap, p, y, res, sc + 1);
+    cg_step_kernel<<<1, 1024, 0, st>>>(n, rho_dev, u, ap, p, y, res, sc + 1);
     CC_LAUNCH_CHECK();
   }
   if ((rc = project(y))) return rc;
   return cc_skinny_f64(st, n_env, 2, Wt, (int)d, y, t_out);                // t = W^T P y
+}
+
+extern "C" int cc_normalize_f64(void* stream, int64_t n, const double* x, double* y) {
+  if (n <= 0) return CC_OK;
+  lz_normalize_kernel<<<1, 1024, 0, cc_stream(stream)>>>(n, x, y, nullptr);
+  CC_LAUNCH_CHECK();
+  return CC_OK;
 }

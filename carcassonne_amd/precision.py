@@ -153,7 +153,7 @@ class MixedSolver:
             _lib.call("cc_basis_combine_f64", stream(), k, n, ptr(NU), ptr(coeffs), ptr(t))
             _lib.call("cc_axpby_f64", stream(), n, -rho, ptr(t), 1.0, ptr(r))
             _lib.call("cc_jd_correction_lp", stream(), self.Hlp.plan, f.n_env, f.d, ptr(f.Lt), ptr(f.W), ptr(f.Wt),
-                      float(rho), ptr(xr), ptr(r), int(inner_iters), ptr(self._jd_work), ptr(t))
+                      ptr(small[0:1]), ptr(xr), ptr(r), int(inner_iters), ptr(self._jd_work), ptr(t))
             # orthogonalise the correction against the basis (twice, Euclidean) and normalise
             for _ in range(2):
                 c = empty((k,))
@@ -166,3 +166,49 @@ class MixedSolver:
             _lib.call("cc_axpby_f64", stream(), n, 1.0 / np.sqrt(float(nrm.item())), ptr(t), 0.0, ptr(U[k]))
         return xr, rho, dict(lowprec_iterations=lp_iters, outer_steps=k, fp64_passes=fp64_passes,
                              lowprec_ritz=float(res[0]), lowprec_residual=float(res[1]), history=history)
+
+    def solve_fixed(self, x0, lp_iters=60, outer=3, inner_iters=20):
+        """The same solve with a fixed amount of work and no host reads or synchronisation (so it can be
+        captured in a HIP graph): lp_iters low-precision Lanczos steps, then `outer` FP64 Rayleigh-Ritz
+        passes with outer - 1 corrections between them.  Returns (x, rho) as device tensors; rho[0] is
+        the FP64 Rayleigh-Ritz value of x."""
+        from .data import HipData
+        if isinstance(x0, HipData):
+            x0 = x0._t.reshape(-1)
+        f, n = self.factor, self.n
+        lp_iters = int(min(lp_iters, n))
+        if self._lz_work is None or self._lz_max < lp_iters:
+            self._lz_work = empty((_lib.query("cc_lanczos_work_size", n, lp_iters),))
+            self._lz_max = lp_iters
+        x = empty((n,))
+        _lib.call("cc_lanczos_lp", stream(), self.Hlp.plan, f.n_env, f.d, ptr(f.L), ptr(f.Lt), ptr(f.W), ptr(f.Wt),
+                  ptr(x0), 0.0, 0.0, lp_iters, 0, ptr(self._lz_work), ptr(x), None)
+        kmax = outer
+        U, HU, NU = empty((kmax, n)), empty((kmax, n)), empty((kmax, n))
+        small = empty((kmax + 1,))
+        xr, r, t, tmp = empty((n,)), empty((n,)), empty((n,)), empty((n,))
+        U[0].copy_(x)
+        for k in range(1, outer + 1):
+            _lib.call("cc_stage3_plan_apply", stream(), self._hplan, ptr(U[k - 1]), ptr(HU[k - 1]))
+            _lib.call("cc_stage3_plan_apply", stream(), self._nplan, ptr(U[k - 1]), ptr(NU[k - 1]))
+            hs, ns = empty((k, k)), empty((k, k))
+            for j in range(k):
+                _lib.call("cc_basis_dots_f64", stream(), k, n, ptr(U), ptr(HU[j]), ptr(hs[:, j]), k)
+                _lib.call("cc_basis_dots_f64", stream(), k, n, ptr(U), ptr(NU[j]), ptr(ns[:, j]), k)
+            _lib.call("cc_small_geneig_f64", stream(), k, ptr(hs), ptr(ns), ptr(small[0:1]), ptr(small[1:1 + k]))
+            coeffs = small[1:1 + k]
+            _lib.call("cc_basis_combine_f64", stream(), k, n, ptr(U), ptr(coeffs), ptr(xr))
+            if k == outer:
+                break
+            _lib.call("cc_basis_combine_f64", stream(), k, n, ptr(HU), ptr(coeffs), ptr(r))
+            _lib.call("cc_basis_combine_f64", stream(), k, n, ptr(NU), ptr(coeffs), ptr(tmp))
+            _lib.call("cc_axpby_dev_f64", stream(), n, ptr(small[0:1]), -1.0, ptr(tmp), 1.0, ptr(r))
+            _lib.call("cc_jd_correction_lp", stream(), self.Hlp.plan, f.n_env, f.d, ptr(f.Lt), ptr(f.W), ptr(f.Wt),
+                      ptr(small[0:1]), ptr(xr), ptr(r), int(inner_iters), ptr(self._jd_work), ptr(t))
+            for _ in range(2):
+                c = empty((k,))
+                _lib.call("cc_basis_dots_f64", stream(), k, n, ptr(U), ptr(t), ptr(c), 1)
+                _lib.call("cc_basis_combine_f64", stream(), k, n, ptr(U), ptr(c), ptr(tmp))
+                _lib.call("cc_axpby_f64", stream(), n, -1.0, ptr(tmp), 1.0, ptr(t))
+            _lib.call("cc_normalize_f64", stream(), n, ptr(t), ptr(U[k]))
+        return xr, small[0:1]

@@ -57,6 +57,10 @@ int cc_gemm_f64(void* stream, int64_t batch, int64_t M, int64_t N, int64_t K, do
 
 /* y = alpha * x + beta * y.   NDArrayData.__add__/__iadd__/__mul__  data/__init__.py:93-139 */
 int cc_axpby_f64(void* stream, int64_t n, double alpha, const double* x, double beta, double* y);
+/* y = (alpha_scale * alpha_dev[0]) * x + beta * y with the coefficient read from device memory (solver steps
+ * that must not synchronise with the host).  Same arithmetic as cc_axpby_f64. */
+int cc_axpby_dev_f64(void* stream, int64_t n, const double* alpha_dev, double alpha_scale, const double* x, double beta,
+                     double* y);
 
 /* y[i] = value.   newZeros / newFilled / newTrivial  data/__init__.py:51-90 */
 int cc_fill_f64(void* stream, int64_t n, double value, double* y);
@@ -159,7 +163,10 @@ int cc_skinny_f64(void* stream, int64_t n, int tri, const double* M, int nrhs, c
  * N = (L L^T) (x) I_d, on the whitened operator W H W^T (W = L^-1).  H is a stage-3 plan.
  * Stops when the Ritz residual <= tol_residual*|theta|, when the Ritz value moved by at most
  * tol_energy*|theta| since the previous check (every check_every steps), or at max_iter.
- * x_out is N-normalised; host_result = {theta, residual estimate, iterations}.
+ * x_out is N-normalised; host_result = {theta, residual estimate, iterations}.  check_every = 0 runs
+ * exactly max_iter steps (fewer if the Krylov space completes) without any host read or
+ * synchronisation, so the call can be captured in a HIP graph; host_result (may be null) then receives
+ * NaN for theta and the residual, which stay in device memory.
  * Replaces relaxOver (utils.py:805-878) as the residual-controlled solver of
  * minimizeExpectation (system/_2d.py:489-497). */
 int cc_lanczos_work_size(int64_t n, int max_iter, int64_t* out);
@@ -199,14 +206,16 @@ int cc_lanczos_lp(void* stream, const void* lp_plan, int64_t n_env, int64_t d, c
  * out[j*stride_out] = U[j] . v for j < k;  out = sum_j c[j] U[j];  lowest eigenpair of the k x k
  * pencil (Hs, Ns), k <= 32, out[0] = value, vec = N-normalised coefficients;  the correction
  * t = W^T P y with y from `iters` CG steps on P (W H_lp W^T - rho) P y = -P W r (P = I - u u^T,
- * u = L^T x / |L^T x|); `work` holds cc_jd_work_size doubles. */
+ * u = L^T x / |L^T x|, rho = rho_dev[0] read on the device); `work` holds cc_jd_work_size doubles. */
 int cc_basis_dots_f64(void* stream, int k, int64_t n, const double* U, const double* v, double* out, int stride_out);
 int cc_basis_combine_f64(void* stream, int k, int64_t n, const double* U, const double* c, double* out);
 int cc_small_geneig_f64(void* stream, int k, const double* Hs, const double* Ns, double* out, double* vec);
 int cc_jd_work_size(int64_t n, int64_t* out);
 int cc_jd_correction_lp(void* stream, const void* lp_plan, int64_t n_env, int64_t d, const double* Lt,
-                        const double* W, const double* Wt, double rho, const double* x, const double* r,
+                        const double* W, const double* Wt, const double* rho_dev, const double* x, const double* r,
                         int iters, double* work, double* t_out);
+/* y = x / |x| (one workgroup, fixed-order sum; the Lanczos normalisation). */
+int cc_normalize_f64(void* stream, int64_t n, const double* x, double* y);
 
 /* ---- learned warm start: a 3-layer MLP proposing the eigensolver's start vector ----------------
  * (no reference counterpart: the reference always starts relaxOver from the current center,
