@@ -15,12 +15,12 @@ from .base import BaseSystem
 from ..compression import computeProductCompressor
 from ..data import HipData
 from ..errors import InvariantViolatedError
-from ..sparse import (Identity, OneSiteOperator, TwoSiteOperator, makeSimpleSparseOperator, makeSparseOperator,
-                      mapOverSparseData, stripAllButIdentityFrom)
+from ..sparse import (Identity, OneSiteOperator, TwoSiteOperator, TwoSiteOperatorCompressed, makeSimpleSparseOperator,
+                      makeSparseOperator, mapOverSparseData, stripAllButIdentityFrom)
 from ..tensors._2d.dense import formNormalizationMultiplier, formNormalizationSubmatrix
 from ..tensors._2d.sparse import (absorbSparseCenterSOSIntoSide, absorbSparseSideIntoCornerFromLeft,
                                   absorbSparseSideIntoCornerFromRight, formExpectationAndNormalizationMultipliers)
-from ..utils import L, O, R, relaxOver
+from ..utils import L, O, R, Multiplier, computeCompressor, relaxOver
 
 
 def sideFromCorner(corner_id, direction):
@@ -195,6 +195,77 @@ class System(BaseSystem):
                                                     self.corners[corner_id])
         self.sides[corner_id] = mapOverSparseData(lambda t: t.absorbMatrixAt(0, U.conj()).absorbMatrixAt(1, U),
                                                   self.sides[corner_id])
+
+    def compressCornerTwoSiteOperatorTowards(self, corner_id, direction, new_dimension, normalize=False):
+        """_2d.py:229-363: fold the corner's open two-site channels of `direction` and its compressed
+        channel into one TwoSiteOperatorCompressed(direction) channel of new_dimension along the
+        operator leg (axis 3*direction + 2), and the matching channels of the side it meets into
+        TwoSiteOperatorCompressed(1 - direction).  The Gram matrix is block diagonal (open channels |
+        compressed channel) as in the reference; the open channels are reshaped to their own count (the
+        reference reshapes to the total old dimension, which only works before any compression)."""
+        axis = 3 * direction + 2
+        new_corner, sparse, pos_map, old_comp = {}, [], {}, None
+        for tag, data in self.corners[corner_id].items():
+            if isinstance(tag, TwoSiteOperator) and tag.direction == direction:
+                assert tag.position not in pos_map
+                pos_map[tag.position] = len(sparse)
+                sparse.append(data)
+            elif isinstance(tag, TwoSiteOperatorCompressed) and tag.direction == direction:
+                assert old_comp is None
+                old_comp = data
+            else:
+                new_corner[tag] = data
+        n1 = len(sparse)
+        old_dimension = n1 + (old_comp.shape[axis] if old_comp is not None else 0)
+        if old_dimension == 0:
+            return
+        blocks = []
+        if n1:
+            collected = type(sparse[0]).newCollected(sparse).dropUnitAxis(axis + 1)  # (n1, others)
+            flat = collected.join(0, list(range(1, collected.ndim)))
+            blocks.append(flat.conj().contractWith(flat, (1,), (1,)))
+        if old_comp is not None:
+            f = old_comp.fold(axis)
+            blocks.append(f.conj().contractWith(f, (1,), (1,)))
+        M = blocks[0] if len(blocks) == 1 else blocks[0].directSumWith(blocks[1])
+        corner_mult, side_mult_conj = computeCompressor(
+            old_dimension, new_dimension,
+            Multiplier((old_dimension,) * 2, lambda v: M.contractWith(v, (1,), (0,)), old_dimension ** 2,
+                       lambda: M, 0),
+            None, normalize)
+        side_mult = side_mult_conj.conj()
+
+        def compress(part, mult, comp_old, ax):
+            out = None
+            if part is not None:
+                t = mult[:, :n1].contractWith(part, (1,), (0,))                       # (new, others)
+                perm = list(range(1, t.ndim))
+                perm.insert(ax, 0)
+                out = t.transpose(perm)
+            if comp_old is not None:
+                t2 = comp_old.absorbMatrixAt(ax, mult[:, n1:])
+                out = t2 if out is None else out + t2
+            return out
+
+        new_corner[TwoSiteOperatorCompressed(direction)] = compress(collected if n1 else None, corner_mult,
+                                                                    old_comp, axis)
+        self.corners[corner_id] = new_corner
+        side_id = sideFromCorner(corner_id, direction)
+        sdir = 1 - direction
+        saxis = 3 * sdir + 2
+        parts, old_side, new_side = [None] * n1, None, {}
+        for tag, data in self.sides[side_id].items():
+            if isinstance(tag, TwoSiteOperator) and tag.direction == sdir:
+                parts[pos_map[tag.position]] = data
+            elif isinstance(tag, TwoSiteOperatorCompressed) and tag.direction == sdir:
+                assert old_comp is not None and old_side is None
+                old_side = data
+            else:
+                new_side[tag] = data
+        assert all(p is not None for p in parts)
+        spart = type(parts[0]).newCollected(parts).dropUnitAxis(saxis + 1) if n1 else None
+        new_side[TwoSiteOperatorCompressed(sdir)] = compress(spart, side_mult, old_side, saxis)
+        self.sides[side_id] = new_side
 
     # -- observables (_2d.py:364-431) ---------------------------------------------------------
     def computeScalarUsingMultiplier(self, multiply):

@@ -266,3 +266,35 @@ def relaxOver(*args, **kwargs):
     """utils.py:805-878 on device vectors (see carcassonne_amd.solver.relax_over)."""
     from .solver import relax_over
     return relax_over(*args, **kwargs)
+
+
+def computeCompressor(old_dimension, new_dimension, multiplier, dtype=None, normalize=False):
+    """utils.py:268-303: the eigenvectors of the new_dimension largest eigenvalues of the (Hermitian
+    positive semi-definite) Gram matrix behind `multiplier`, as the rows of the compressor
+    (new_dimension x old_dimension), and the inverse compressor's conjugate (the same rows, or with
+    normalize the rows scaled by sqrt(eval) and 1/sqrt(eval)).  The matrix is formed and decomposed on
+    the device (one-sided Jacobi SVD); the reference forms it when new >= old/2 and otherwise runs
+    ARPACK on the same eigenpairs."""
+    if new_dimension < 0:
+        raise ValueError("New dimension ({}) must be non-negative.".format(new_dimension))
+    if new_dimension > old_dimension:
+        raise ValueError("New dimension ({}) must be less than or equal to the old dimension ({})."
+                         .format(new_dimension, old_dimension))
+    matrix = multiplier.formMatrix()
+    if new_dimension == 0:
+        z = type(matrix).newZeros((0, old_dimension))
+        return z, z
+    if tuple(matrix.shape) != (old_dimension,) * 2:
+        raise ValueError("Multiplier matrix has shape {} but the old dimension is {}.".format(matrix.shape, old_dimension))
+    # M = V S V^T for a PSD matrix: the rows of V^T are eigenvectors, largest first.  (The left factor
+    # is not used: one-sided Jacobi leaves the columns of U for zero singular values undefined, while
+    # V, the product of the rotations, is orthogonal whatever the rank.)
+    _, S, Vt = matrix.svd(False)
+    evals = S.toArray()[:new_dimension]
+    if abs(evals[0]) < 1e-15:
+        raise ValueError("Input is filled with near-zero elements.")
+    evecs = Vt[:new_dimension, :]
+    if normalize:
+        r = type(matrix).fromArray(np.sqrt(evals).reshape(new_dimension, 1) * np.ones((1, old_dimension)))
+        return evecs * r, evecs / r
+    return evecs, evecs

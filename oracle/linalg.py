@@ -5,13 +5,17 @@
 * unitize           -- polar factor U V^H (carcassonne/utils.py:879-882)
 * new_enlargener    -- NDArrayData.newEnlargener (data/__init__.py:43-50)
 * product_compressor-- computeProductCompressor (carcassonne/compression.py:11-46)
+* compute_compressor -- computeCompressor (carcassonne/utils.py:268-303): the leading eigenvectors
+                       of a Gram matrix (operator-channel compression)
 
 Pinned by: normalize_axis -- isometry and inverse identities
 (reference test_data.py:98-133 restated in tests/test_oracle_linalg.py);
 product_compressor -- exact reconstruction when the target rank suffices
 (reference test_compression.py:9-30, test_system.py:40-81).  The raw compressor
 after four non-converged ALS sweeps is "parity unpinned" (the reference starts
-from an unseeded random isometry and uses SciPy GMRES defaults).
+from an unseeded random isometry and uses SciPy GMRES defaults).  compute_compressor --
+eigenvalue identities of its output and the operator-compression exactness of
+tests/test_oracle_system.py (reference test_system.py:82-177).
 """
 import numpy as np
 from scipy.linalg import svd
@@ -123,3 +127,29 @@ def product_compressor(Lt, Rt, new_dimension, rng, sweeps=4):
         x = np.linalg.lstsq(A, b, rcond=None)[0]
         U = unitize(x.reshape(old, new_dimension))
     return U.T
+
+
+def compute_compressor(old_dimension, new_dimension, matrix, normalize=False):
+    """computeCompressor (utils.py:268-303) on an explicit Hermitian PSD Gram matrix: the eigenvectors of
+    the new_dimension largest eigenvalues (eigh; the reference switches to ARPACK eigsh when
+    new < old/2, which targets the same eigenpairs), as rows of the compressor; with normalize the rows
+    are scaled by sqrt(eval) and the inverse by 1/sqrt(eval).  Returns (compressor, inverse_conj),
+    both new x old."""
+    if new_dimension < 0:
+        raise ValueError("New dimension ({}) must be non-negative.".format(new_dimension))
+    if new_dimension > old_dimension:
+        raise ValueError("New dimension ({}) must be less than or equal to the old dimension ({})."
+                         .format(new_dimension, old_dimension))
+    if new_dimension == 0:
+        z = np.zeros((0, old_dimension))
+        return z, z
+    from scipy.linalg import eigh
+    evals, evecs = eigh(matrix)
+    evals, evecs = evals[-new_dimension:], evecs[:, -new_dimension:]
+    if abs(evals[-1]) < 1e-15:
+        raise ValueError("Input is filled with near-zero elements.")
+    evecs = evecs.T
+    if normalize:
+        r = np.sqrt(evals).reshape(new_dimension, 1)
+        return evecs * r, evecs / r
+    return evecs, evecs

@@ -18,7 +18,7 @@ import numpy as np
 from scipy.linalg import eigh
 
 from . import dense, linalg, solver, sparse2d
-from .tags import (Identity, OneSiteOperator, TwoSiteOperator, L, R, O,
+from .tags import (Identity, OneSiteOperator, TwoSiteOperator, TwoSiteOperatorCompressed, L, R, O,
                    make_simple_sparse_operator, make_sparse_operator, strip_all_but_identity)
 
 
@@ -160,6 +160,74 @@ class System:
         U = linalg.product_compressor(corner, side, new_dimension, self.rng)
         self.corners[c] = _map(lambda t: linalg.absorb_matrix_at(linalg.absorb_matrix_at(t, 3, U), 4, U.conj()), self.corners[c])
         self.sides[c] = _map(lambda t: linalg.absorb_matrix_at(linalg.absorb_matrix_at(t, 0, U.conj()), 1, U), self.sides[c])
+
+    def compressCornerTwoSiteOperatorTowards(self, corner_id, direction, new_dimension, normalize=False):
+        """_2d.py:229-363: compress the corner's open two-site channels of `direction` (and its existing
+        compressed channel) into one TwoSiteOperatorCompressed(direction) channel of new_dimension
+        along the operator leg, with the matching side channels.  The Gram matrix is block diagonal
+        (open channels | compressed channel), as in the reference.  The open channels are reshaped to
+        their own count (the reference reshapes them to the total old dimension, which only works when
+        no compressed channel exists yet)."""
+        axis = 3 * direction + 2
+        new_corner, sparse, pos_map, old_comp = {}, [], {}, None
+        for tag, data in self.corners[corner_id].items():
+            if isinstance(tag, TwoSiteOperator) and tag.direction == direction:
+                assert tag.position not in pos_map
+                pos_map[tag.position] = len(sparse)
+                sparse.append(data)
+            elif isinstance(tag, TwoSiteOperatorCompressed) and tag.direction == direction:
+                assert old_comp is None
+                old_comp = data
+            else:
+                new_corner[tag] = data
+        n1 = len(sparse)
+        old_dimension = n1 + (old_comp.shape[axis] if old_comp is not None else 0)
+        if old_dimension == 0:
+            return
+        collected = np.array([d.ravel() for d in sparse]).reshape(n1, -1)
+        m1 = collected.conj() @ collected.T
+        if old_comp is not None:
+            f = linalg.join(old_comp, axis, [i for i in range(old_comp.ndim) if i != axis])
+            m2 = f.conj() @ f.T
+        else:
+            m2 = np.zeros((0, 0))
+        M = np.zeros((old_dimension, old_dimension), dtype=np.result_type(m1, m2))
+        M[:n1, :n1] = m1
+        M[n1:, n1:] = m2
+        corner_mult, side_mult_conj = linalg.compute_compressor(old_dimension, new_dimension, M, normalize)
+        side_mult = side_mult_conj.conj()
+        shape = list(self.corners[corner_id][Identity()].shape)
+        del shape[axis]
+
+        def compress(part, mult, comp_old, ax):
+            out = None
+            if part is not None:
+                t = np.tensordot(mult[:, :n1], part, (1, 0))          # (new, others)
+                out = np.moveaxis(t, 0, ax)
+            if comp_old is not None:
+                t2 = linalg.absorb_matrix_at(comp_old, ax, mult[:, n1:])
+                out = t2 if out is None else out + t2
+            return out
+
+        part = collected.reshape([n1] + shape) if n1 else None
+        new_corner[TwoSiteOperatorCompressed(direction)] = compress(part, corner_mult, old_comp, axis)
+        self.corners[corner_id] = new_corner
+        side_id = side_from_corner(corner_id, direction)
+        sdir = 1 - direction
+        saxis = 3 * sdir + 2
+        parts, old_side, new_side = [None] * n1, None, {}
+        for tag, data in self.sides[side_id].items():
+            if isinstance(tag, TwoSiteOperator) and tag.direction == sdir:
+                parts[pos_map[tag.position]] = data
+            elif isinstance(tag, TwoSiteOperatorCompressed) and tag.direction == sdir:
+                assert old_comp is not None and old_side is None
+                old_side = data
+            else:
+                new_side[tag] = data
+        assert all(p is not None for p in parts)
+        spart = np.array([np.squeeze(d, axis=saxis) for d in parts]) if n1 else None
+        new_side[TwoSiteOperatorCompressed(sdir)] = compress(spart, side_mult, old_side, saxis)
+        self.sides[side_id] = new_side
 
     # -- operators and observables (_2d.py:367-431,455-459) -----------------------------
     def formOperator(self, op_center=None):

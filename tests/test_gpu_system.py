@@ -200,3 +200,63 @@ def test_2d_in_1d_ferromagnet_known_answer(gpu):
     system.setPolicy("contraction", P.RepeatPatternContractionPolicy([0, 2]))
     system.runUntilConverged(maximum_number_of_runs=20, maximum_number_of_steps=100)
     assert round(system.computeOneSiteExpectation() - (-1.0), 7) == 0
+
+
+def _operator_systems(corner_id, direction, seed):
+    """The same two-site-term system in the oracle and on the GPU (reference test_system.py:82-177)."""
+    from carcassonne_amd.data import HipData
+    from carcassonne_amd.system import System
+    from carcassonne_amd.system._2d import sideFromCorner
+    from carcassonne_amd.utils import L, R
+    from oracle.system import System as OSystem
+    rng = np.random.default_rng(seed)
+    ops = []
+    for _ in range(2):
+        a = rng.standard_normal((2, 2))
+        ops.append(a + a.T)
+    side_id = sideFromCorner(corner_id, direction)
+    key = "OO_LR" if side_id in (1, 3) else "OO_UD"
+    gs = System.newTrivialWithSimpleSparseOperator(**{key: [HipData.fromArray(o) for o in ops]})
+    os_ = OSystem.newTrivialWithSimpleSparseOperator(**{key: ops})
+    first = R(side_id) if direction == 0 else L(side_id)
+    for s in (gs, os_):
+        s.contractTowards(first)
+    return gs, os_, side_id
+
+
+@pytest.mark.parametrize("corner_id,direction", [(0, 0), (1, 1), (2, 0), (3, 1)])
+def test_operator_compression_exact_and_matches_oracle(gpu, corner_id, direction):
+    from carcassonne_amd.sparse import TwoSiteOperatorCompressed
+    gs, os_, side_id = _operator_systems(corner_id, direction, 20 + corner_id)
+    for _ in range(4):
+        gs.contractTowards(side_id)
+        os_.contractTowards(side_id)
+    e1, n1 = gs.computeExpectationAndNormalization()
+    gs.compressCornerTwoSiteOperatorTowards(corner_id, direction, 4)
+    os_.compressCornerTwoSiteOperatorTowards(corner_id, direction, 4)
+    assert TwoSiteOperatorCompressed(direction) in gs.corners[corner_id]
+    assert TwoSiteOperatorCompressed(1 - direction) in gs.sides[side_id]
+    e2, n2 = gs.computeExpectationAndNormalization()
+    oe, on = os_.computeExpectationAndNormalization()
+    assert abs(n2 / n1 - 1) < 1e-10 and abs(e2 - e1) < 1e-10 * max(1, abs(e1))
+    assert abs(e2 - oe) < 1e-10 * max(1, abs(oe))
+    # the compressed channels keep working in further growth steps
+    gs.contractTowards(side_id)
+    os_.contractTowards(side_id)
+    assert abs(gs.computeExpectation() - os_.computeExpectation()) < 1e-10 * max(1, abs(os_.computeExpectation()))
+
+
+def test_operator_compression_below_the_term_count(gpu):
+    """More two-site terms than the channel rank: compressing to the rank is exact (reference
+    test_system.py:82-115)."""
+    gs, os_, side_id = _operator_systems(0, 1, 7)
+    for _ in range(2):
+        gs.contractTowards(side_id)
+    state = gs.state_center_data
+    for _ in range(5):
+        gs.contractTowards(side_id)
+    gs.setStateCenter(state)
+    e1, n1 = gs.computeExpectationAndNormalization()
+    gs.compressCornerTwoSiteOperatorTowards(0, 1, 2)
+    e2, n2 = gs.computeExpectationAndNormalization()
+    assert abs(n2 / n1 - 1) < 1e-10 and abs(e2 - e1) < 1e-9 * max(1, abs(e1))

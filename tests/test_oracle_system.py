@@ -216,3 +216,79 @@ def test_synthetic_environment_is_symmetric_definite():
         H, N = op.matrix_H(), op.matrix_N()
         assert np.abs(H - H.T).max() < 1e-12 * np.abs(H).max()
         assert np.linalg.eigvalsh(N).min() > 0
+
+
+def _operator_system(corner_id, direction, d, rng):
+    """reference test_system.py:82-177: two-site terms along the side that corner `corner_id` meets in
+    `direction`, accumulated by contracting towards that side."""
+    side_id = side_from_corner(corner_id, direction)
+    ops = []
+    for _ in range(2):
+        a = rng.standard_normal((d, d))
+        ops.append(a + a.T)
+    kw = {"OO_LR": ops} if side_id in (1, 3) else {"OO_UD": ops}
+    s = System.newTrivialWithSimpleSparseOperator(**kw)
+    s.contractTowards(R(side_id) if direction == 0 else L(side_id))
+    return s, side_id
+
+
+@pytest.mark.parametrize("corner_id", range(4))
+@pytest.mark.parametrize("direction", [0, 1])
+def test_operator_compression_same_dimension_is_exact(corner_id, direction):
+    rng = np.random.default_rng(10 * corner_id + direction)
+    s, side_id = _operator_system(corner_id, direction, 2, rng)
+    n_terms = 4
+    for _ in range(n_terms):
+        s.contractTowards(side_id)
+    from oracle.tags import TwoSiteOperator, TwoSiteOperatorCompressed
+    n_corner = sum(isinstance(t, TwoSiteOperator) for t in s.corners[corner_id])
+    assert n_corner == n_terms
+    e1, n1 = s.computeExpectationAndNormalization()
+    s.compressCornerTwoSiteOperatorTowards(corner_id, direction, n_terms)
+    assert TwoSiteOperatorCompressed(direction) in s.corners[corner_id]
+    assert TwoSiteOperatorCompressed(1 - direction) in s.sides[side_id]
+    e2, n2 = s.computeExpectationAndNormalization()
+    assert abs(n2 / n1 - 1) < 1e-10 and abs(e2 - e1) < 1e-10 * max(1, abs(e1))
+
+
+@pytest.mark.parametrize("corner_id", [0, 3])
+@pytest.mark.parametrize("direction", [0, 1])
+def test_operator_compression_down_by_half_is_exact(corner_id, direction):
+    """More two-site terms than the operator-channel rank: compressing to the rank is exact (the
+    reference's down_by_half test)."""
+    rng = np.random.default_rng(3 + corner_id + direction)
+    s, side_id = _operator_system(corner_id, direction, 2, rng)
+    new_dim = 2
+    for _ in range(new_dim):
+        s.contractTowards(side_id)
+    state = s.state
+    for _ in range(2 * new_dim + 1):
+        s.contractTowards(side_id)
+    s.setStateCenter(state)
+    e1, n1 = s.computeExpectationAndNormalization()
+    s.compressCornerTwoSiteOperatorTowards(corner_id, direction, new_dim)
+    e2, n2 = s.computeExpectationAndNormalization()
+    assert abs(n2 / n1 - 1) < 1e-10 and abs(e2 - e1) < 1e-9 * max(1, abs(e1))
+
+
+
+def test_operator_compression_trivial_system_is_a_no_op():
+    s = System.newTrivialWithSimpleSparseOperator(O=np.identity(1))
+    e1, n1 = s.computeExpectationAndNormalization()
+    for c in range(4):
+        for d in (0, 1):
+            s.compressCornerTwoSiteOperatorTowards(c, d, 3)
+    e2, n2 = s.computeExpectationAndNormalization()
+    assert e1 == e2 and n1 == n2
+
+
+def test_compute_compressor_keeps_the_leading_eigenvectors():
+    from oracle import linalg
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((6, 3))
+    M = a @ a.T                                   # rank 3
+    C, Ci = linalg.compute_compressor(6, 3, M)
+    np.testing.assert_allclose(C @ C.T, np.eye(3), atol=1e-12)
+    np.testing.assert_allclose(C.T @ (C @ M @ C.T) @ C, M, atol=1e-10)   # the rank-3 Gram is reproduced
+    Cn, Cin = linalg.compute_compressor(6, 3, M, normalize=True)
+    np.testing.assert_allclose(Cn @ Cin.T, np.eye(3), atol=1e-12)
