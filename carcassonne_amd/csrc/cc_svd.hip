@@ -6,6 +6,7 @@
 namespace {
 constexpr int SVD_THREADS = 1024;
 constexpr int SVD_MAX_N = 256;
+constexpr double SVD_ZERO = 0x1p-960;  // singular values below this count as zero (W / s loses precision)
 
 __device__ inline double wave_sum(double v) {
 #pragma unroll
@@ -110,11 +111,94 @@ __global__ void __launch_bounds__(SVD_THREADS) svd_jacobi_kernel(int64_t m, int 
     int r = (int)(e % n);
     int j = perm[r];
     double s = snorm[j];
-    U[e] = s > 0.0 ? W[(int64_t)j * m + i] / s : 0.0;
+    U[e] = s >= SVD_ZERO ? W[(int64_t)j * m + i] / s : 0.0;
   }
   for (int e = tid; e < n * n; e += SVD_THREADS) {
     int r = e / n, k = e % n;
     Vt[e] = V[(int64_t)perm[r] * n + k];
+  }
+
+  // Columns of zero singular value carry no direction of A; give them an orthonormal completion so U
+  // always has orthonormal columns (LAPACK's thin U does, and polar factors U Vt must be isometries).
+  // Column r takes the unit vector e_k least covered by the accepted columns (smallest row norm, lowest
+  // k on ties, so |P_perp e_k|^2 >= (m - r) / m), orthogonalised twice and normalised.
+  __shared__ int first_zero;
+  __shared__ double coef[SVD_MAX_N];
+  __shared__ double red_v[SVD_THREADS / 64];
+  __shared__ int64_t red_i[SVD_THREADS / 64];
+  if (tid == 0) {
+    int f = n;
+    while (f > 0 && !(snorm[perm[f - 1]] >= SVD_ZERO)) --f;
+    first_zero = f;
+  }
+  __syncthreads();
+  if (first_zero == n) return;
+  double* rn = W;  // m row norms of U's accepted columns (W is no longer needed)
+  for (int64_t i = tid; i < m; i += SVD_THREADS) {
+    double a = 0.0;
+    for (int c = 0; c < first_zero; ++c) { double u = U[i * n + c]; a += u * u; }
+    rn[i] = a;
+  }
+  __syncthreads();
+  for (int r = first_zero; r < n; ++r) {
+    // k = argmin over rows of rn (lowest index on ties)
+    double bv = INFINITY;
+    int64_t bi = m;
+    for (int64_t i = tid; i < m; i += SVD_THREADS)
+      if (rn[i] < bv) { bv = rn[i]; bi = i; }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      double ov = __shfl_xor(bv, o, 64);
+      int64_t oi = __shfl_xor(bi, o, 64);
+      if (ov < bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+    }
+    if (lane == 0) { red_v[wave] = bv; red_i[wave] = bi; }
+    __syncthreads();
+    if (tid == 0) {
+      for (int w = 1; w < nwaves; ++w)
+        if (red_v[w] < red_v[0] || (red_v[w] == red_v[0] && red_i[w] < red_i[0])) { red_v[0] = red_v[w]; red_i[0] = red_i[w]; }
+    }
+    __syncthreads();
+    const int64_t k = red_i[0] < m ? red_i[0] : 0;  // NaN row norms leave no candidate
+    for (int c = tid; c < r; c += SVD_THREADS) coef[c] = U[k * n + c];
+    __syncthreads();
+    for (int64_t i = tid; i < m; i += SVD_THREADS) {
+      double v = (i == k) ? 1.0 : 0.0;
+      for (int c = 0; c < r; ++c) v -= U[i * n + c] * coef[c];
+      U[i * n + r] = v;
+    }
+    __syncthreads();
+    // second Gram-Schmidt pass: coef[c] = <U_c, v>, one wave per column
+    for (int c = wave; c < r; c += nwaves) {
+      double d = 0.0;
+      for (int64_t i = lane; i < m; i += 64) d += U[i * n + c] * U[i * n + r];
+      d = wave_sum(d);
+      if (lane == 0) coef[c] = d;
+    }
+    __syncthreads();
+    double nn = 0.0;
+    for (int64_t i = tid; i < m; i += SVD_THREADS) {
+      double v = U[i * n + r];
+      for (int c = 0; c < r; ++c) v -= U[i * n + c] * coef[c];
+      U[i * n + r] = v;
+      nn += v * v;
+    }
+    nn = wave_sum(nn);
+    if (lane == 0) red_v[wave] = nn;
+    __syncthreads();
+    if (tid == 0) {
+      double t = 0.0;
+      for (int w = 0; w < nwaves; ++w) t += red_v[w];
+      red_v[0] = 1.0 / sqrt(t);
+    }
+    __syncthreads();
+    const double sc = red_v[0];
+    for (int64_t i = tid; i < m; i += SVD_THREADS) {
+      double v = U[i * n + r] * sc;
+      U[i * n + r] = v;
+      rn[i] += v * v;
+    }
+    __syncthreads();
   }
 }
 }  // namespace

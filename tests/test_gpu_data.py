@@ -63,6 +63,43 @@ def test_svd_and_normalize_axis(gpu):
         np.testing.assert_allclose(iso.toArray(), oiso, rtol=1e-10, atol=1e-12)
 
 
+def test_svd_rank_deficient_has_orthonormal_u(gpu):
+    """Zero singular values: U is completed to orthonormal columns (numpy's thin U is), so polar factors
+    stay isometries; the product U S Vt still reproduces A."""
+    from carcassonne_amd.data import HipData
+    from oracle import linalg as OL
+    rng = _rng(4)
+    a1 = np.zeros((9, 5))
+    a1[:, 1] = rng.standard_normal(9)            # rank 1 with four exactly-zero columns
+    a2 = np.outer(rng.standard_normal(12), rng.standard_normal(6))   # rank 1, rounding-level columns
+    a3 = np.zeros((6, 6))                        # all zero
+    a4 = np.zeros((40, 8))
+    a4[:3, :3] = rng.standard_normal((3, 3))     # block with zero rows and columns
+    a5 = np.zeros((4, 4))
+    a5[0, :] = 1.0                               # row of ones: e_0 is the covered row
+    for a in (a1, a2, a3, a4, a5):
+        m, n = a.shape
+        U, S, Vt = HipData.fromArray(a).svd(False)
+        U, S, Vt = U.toArray(), S.toArray(), Vt.toArray()
+        np.testing.assert_allclose(S, np.linalg.svd(a, compute_uv=False), rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(U.T @ U, np.eye(n), atol=1e-12)
+        np.testing.assert_allclose(Vt @ Vt.T, np.eye(n), atol=1e-12)
+        np.testing.assert_allclose((U * S) @ Vt, a, rtol=1e-11, atol=1e-11)
+        Q = HipData.fromArray(a).unitize().toArray()
+        np.testing.assert_allclose(Q.T @ Q, np.eye(n), atol=1e-12)
+    # normalizeAxis of a tensor whose bond is rank deficient: the isometry is one, the gauge-invariant
+    # normalizer and denormalizer match the oracle
+    s = np.zeros((2, 3, 2, 4))
+    s[..., 0] = rng.standard_normal((2, 3, 2))
+    s[..., 2] = 2.0 * s[..., 0]
+    iso, nrm, den = HipData.fromArray(s).normalizeAxis(3)
+    iso = iso.toArray().reshape(-1, 4)
+    np.testing.assert_allclose(iso.T @ iso, np.eye(4), atol=1e-12)
+    _, onrm, oden = OL.normalize_axis(s, 3)
+    np.testing.assert_allclose(den.toArray(), oden, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(nrm.toArray(), onrm, rtol=1e-10, atol=1e-12)
+
+
 @pytest.mark.parametrize("D,chi,model", [(2, 8, "ising"), (3, 6, "heisenberg"), (4, 8, "ising")])
 def test_stage3_operator_matches_oracle(gpu, D, chi, model):
     from carcassonne_amd import synthetic as syn
