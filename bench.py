@@ -25,6 +25,19 @@ sys.path.insert(0, ROOT)
 WORKLOAD = dict(D=6, chi=64, d=2, model="ising", h=3.0, seed=0)
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+COPY_CALIBRATION_DOUBLES = 64 << 20   # 512 MiB read + 512 MiB written per launch
+
+
+def measured_traffic(kernel_key):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc summaries (FETCH_SIZE and
+    WRITE_SIZE, each scaled by what the counter reads for a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)[kernel_key]
+    except (OSError, KeyError, ValueError):
+        return None
+    return 1024.0 * (t["fetch_kib_per_launch"] * t["fetch_factor"] + t["write_kib_per_launch"] * t["write_factor"])
 
 
 def golden_energy():
@@ -80,6 +93,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--form", choices=["both", "mixed", "fp64"], default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first launch a plain copy kernel (unary_kernel, scale by 1) over 512 MiB three times: "
+                         "its FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic of the solve's kernels")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the solve is a single-GPU kernel path "
@@ -105,6 +121,14 @@ def main():
     x0 = HipData.fromArray(st)
     synchronize()
     e_ref = golden_energy()
+    if args.copy_calibration:
+        src = torch.ones(COPY_CALIBRATION_DOUBLES, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            _lib.call("cc_unary_f64", torch.cuda.current_stream().cuda_stream, 3, src.numel(), src.data_ptr(), 1.0,
+                      dst.data_ptr())
+        synchronize()
+        del src, dst
 
     def run(fn, steps, warmup):
         for _ in range(warmup):
@@ -166,7 +190,8 @@ def main():
         if "fp64" in out:
             line["speedup_vs_plain_fp64"] = out["fp64"]["ms"] / m["ms"]
         line["roofline"] = {"bound": "hbm", "achieved": m["gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                            "frac": m["gbs"] / PEAK_HBM_GBS, "traffic": None,
+                            "frac": m["gbs"] / PEAK_HBM_GBS,
+                            "traffic": measured_traffic("lp_kernel<36,2,f16,split> D=6 chi=64"),
                             "kernel": "lp_kernel<36,2,f16,split> (stage-3 matvec, FP16 slabs)",
                             "algorithmic_bytes_per_launch": m["bytes"]}
     else:
