@@ -50,6 +50,7 @@ SIGNATURES = {
     "cc_linalg_work_size": [_i64, _pi64],
     "cc_cholesky_f64": [_vp, _i64, _vp, _vp, _vp],
     "cc_trtri_lower_f64": [_vp, _i64, _vp, _vp, _vp],
+    "cc_cholesky_inverse_f64": [_vp, _i64, _vp, _vp, _vp, _vp],
     "cc_transpose_f64": [_vp, _i64, _vp, _vp],
     "cc_skinny_f64": [_vp, _i64, _i32, _vp, _i32, _vp, _vp],
     "cc_lanczos_work_size": [_i64, _i32, _pi64],

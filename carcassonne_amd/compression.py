@@ -56,11 +56,10 @@ def _solve_normal_equations(A, b, cutoff=1e-12):
     Lm = (G + shift).tensor().clone()
     info = torch.zeros(1, dtype=torch.int32, device=Lm.device)
     work = empty((_lib.query("cc_linalg_work_size", n),))
-    _lib.call("cc_cholesky_f64", stream(), n, ptr(Lm), ptr(info), ptr(work))
+    W = empty((n, n))
+    _lib.call("cc_cholesky_inverse_f64", stream(), n, ptr(Lm), ptr(W), ptr(info), ptr(work))
     if int(info.item()) != 0:
         raise ValueError("normal equations of the compressor are not positive definite")
-    W = empty((n, n))
-    _lib.call("cc_trtri_lower_f64", stream(), n, ptr(Lm), ptr(W), ptr(work))
     Wt = empty((n, n))
     _lib.call("cc_transpose_f64", stream(), n, ptr(W), ptr(Wt))
     t, x = empty((n,)), empty((n,))

@@ -6,121 +6,118 @@
 namespace {
 constexpr int CB = 64;  // Cholesky block size
 
-// Cholesky of the kb x kb diagonal block at (k0, k0) in LDS on one 256-thread workgroup (kb <= CB; the
-// block is padded to CB with the identity).  Step k updates the trailing lower triangle with the
-// unscaled column k and the pivot (a[i][j] -= a[i][k] a[j][k] / a[k][k]), one barrier per step; the
-// columns are scaled at the end.  The factor is then inverted in place column by column from the last
-// (lower-triangular TRTI2 order: column j of L^-1 below the diagonal is -L^-1[j+1:, j+1:] L[j+1:, j] /
-// L[j][j]), four threads per row sharing each dot product.  Writes L back to A and L^-1 to dinv
-// (kb x kb row-major); info receives the first non-positive pivot (1-based, global index).
-__global__ void __launch_bounds__(256) chol_diag_kernel(int64_t n, int64_t k0, int kb, double* __restrict__ A,
-                                                       double* __restrict__ dinv, int* __restrict__ info) {
-  __shared__ double a[CB][CB + 1];
-  __shared__ double piv[CB];
-  __shared__ double col[CB];
-  const int t = threadIdx.x;
-  for (int e = t; e < CB * CB; e += 256) {
-    const int i = e / CB, j = e % CB;
-    a[i][j] = (i < kb && j <= i) ? A[(k0 + i) * n + k0 + j] : (i == j ? 1.0 : 0.0);
-  }
-  __syncthreads();
-  int bad = 0;
-  const int row = t >> 2, part = t & 3;  // four threads per row of the trailing triangle
-  for (int k = 0; k < CB; ++k) {
-    double d = a[k][k];
-    if (!(d > 0.0)) {
-      if (k < kb && !bad) bad = k + 1;
-      d = 1e-300;
+// Lane l's value of v (the two 32-bit halves through readlane; l uniform).
+__device__ inline double readlane_f64(double v, int l) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u & 0xffffffffull), l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+constexpr int CH = 8, NCH = CB / CH;  // broadcast operands are read from LDS in chunks of CH
+
+// r[j] -= s * src(j) for every j > K.  src is read in chunks of CH, one chunk ahead of its FMAs; the
+// address of each chunk's loads depends (through an empty asm that takes the FMA results) on the chunk
+// two back, so at most two chunks of loaded values are live whatever order the scheduler picks (left to
+// itself it issues all 63 loads of a step first and spills).  K is a template parameter so that every
+// index into r is a constant and r stays in registers.
+template <int K, typename F>
+__device__ __forceinline__ void tri_sweep(double (&r)[CB], double s, F src) {
+  constexpr int c0 = (K + 1) / CH;
+  if constexpr (c0 < NCH) {
+    int tok = 0;
+    double cur[CH], nxt[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) cur[u] = src(c0 * CH + u);
+#pragma unroll
+    for (int c = c0; c < NCH; ++c) {
+#pragma unroll
+      for (int u = 0; u < CH; ++u) nxt[u] = (c + 1 < NCH) ? src((c + 1) * CH + u + tok) : 0.0;
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+        if (c * CH + u > K) r[c * CH + u] -= s * cur[u];
+      __asm__ volatile("" : "+v"(tok) : "v"(r[c * CH + 0]), "v"(r[c * CH + 1]), "v"(r[c * CH + 2]), "v"(r[c * CH + 3]),
+                       "v"(r[c * CH + 4]), "v"(r[c * CH + 5]), "v"(r[c * CH + 6]), "v"(r[c * CH + 7]));
+#pragma unroll
+      for (int u = 0; u < CH; ++u) cur[u] = nxt[u];
     }
-    if (t == 0) piv[k] = d;
-    if (row > k) {
-      const double l = a[row][k] / d;
-#pragma unroll 4
-      for (int j = k + 1 + part; j <= row; j += 4) a[row][j] -= l * a[j][k];
-    }
-    __syncthreads();
-  }
-  if (t == 0 && bad && *info == 0) *info = (int)(k0 + bad);
-  // L[i][j] = a[i][j] / sqrt(piv[j]) below the diagonal, sqrt(piv[i]) on it
-  for (int e = t; e < CB * CB; e += 256) {
-    const int i = e / CB, j = e % CB;
-    if (j < i) a[i][j] *= 1.0 / sqrt(piv[j]);
-    else if (j == i) a[i][i] = sqrt(piv[i]);
-  }
-  __syncthreads();
-  for (int e = t; e < kb * kb; e += 256) {
-    const int i = e / kb, j = e % kb;
-    A[(k0 + i) * n + k0 + j] = (j <= i) ? a[i][j] : 0.0;
-  }
-  __syncthreads();
-  // in-place inverse, columns from the last; rows i > j: 4 lanes per row split the dot product
-  for (int j = CB - 1; j >= 0; --j) {
-    const double ljj = a[j][j];
-    double acc = 0.0;
-    if (row > j) {
-      double acc2 = 0.0;
-      int k = j + 1 + part;
-      for (; k + 4 <= row; k += 8) {  // two independent chains, loads issued ahead
-        acc += a[row][k] * a[k][j];
-        acc2 += a[row][k + 4] * a[k + 4][j];
-      }
-      if (k <= row) acc += a[row][k] * a[k][j];
-      acc += acc2;
-    }
-    acc += __shfl_xor(acc, 1, 64);
-    acc += __shfl_xor(acc, 2, 64);
-    if (part == 0 && row > j) col[row] = -acc / ljj;
-    __syncthreads();
-    if (part == 0 && row > j) a[row][j] = col[row];
-    if (t == 0) a[j][j] = 1.0 / ljj;
-    __syncthreads();
-  }
-  for (int e = t; e < kb * kb; e += 256) {
-    const int i = e / kb, j = e % kb;
-    dinv[i * kb + j] = (j <= i) ? a[i][j] : 0.0;
   }
 }
 
-// Inverse of every diagonal block of L into X (block g on workgroup g): in-place TRTI2 order in LDS as in
-// chol_diag_kernel (columns from the last, four threads per row per dot product).
-__global__ void __launch_bounds__(256) diag_block_inverse_kernel(int64_t n, const double* __restrict__ L,
-                                                                 double* __restrict__ X) {
-  __shared__ double a[CB][CB + 1];
-  __shared__ double col[CB];
-  const int64_t k0 = (int64_t)blockIdx.x * CB;
-  const int kb = (int)((n - k0) < CB ? (n - k0) : CB);
-  const int t = threadIdx.x;
-  for (int e = t; e < CB * CB; e += 256) {
-    const int i = e / CB, j = e % CB;
-    a[i][j] = (i < kb && j <= i) ? L[(k0 + i) * n + k0 + j] : (i == j ? 1.0 : 0.0);
-  }
-  __syncthreads();
-  const int row = t >> 2, part = t & 3;
-  for (int j = CB - 1; j >= 0; --j) {
-    const double ljj = a[j][j];
-    double acc = 0.0;
-    if (row > j) {
-      double acc2 = 0.0;
-      int k = j + 1 + part;
-      for (; k + 4 <= row; k += 8) {  // two independent chains, loads issued ahead
-        acc += a[row][k] * a[k][j];
-        acc2 += a[row][k + 4] * a[k + 4][j];
-      }
-      if (k <= row) acc += a[row][k] * a[k][j];
-      acc += acc2;
+// Right-looking Cholesky steps K..CB-1 with lane i owning row i: the pivot comes from lane K (readlane),
+// column K is scaled by 1/sqrt(pivot), broadcast through LDS, and each lane updates its row.  A
+// non-positive pivot of a real row records its 1-based index in bad and is replaced by 1e-300.
+template <int K>
+__device__ __forceinline__ void chol_steps(double (&r)[CB], double* colv, int i, int kb, int& bad) {
+  if constexpr (K < CB) {
+    double d = readlane_f64(r[K], K);
+    if (!(d > 0.0)) {
+      if (K < kb && bad == 0) bad = K + 1;
+      d = 1e-300;
     }
-    acc += __shfl_xor(acc, 1, 64);
-    acc += __shfl_xor(acc, 2, 64);
-    if (part == 0 && row > j) col[row] = -acc / ljj;
+    const double sd = sqrt(d), inv = 1.0 / sd;
+    const double l = (i > K) ? r[K] * inv : 0.0;
+    r[K] = (i == K) ? sd : ((i > K) ? l : r[K]);
+    colv[i] = l;
     __syncthreads();
-    if (part == 0 && row > j) a[row][j] = col[row];
-    if (t == 0) a[j][j] = 1.0 / ljj;
+    tri_sweep<K>(r, l, [&](int j) { return colv[j]; });
     __syncthreads();
+    chol_steps<K + 1>(r, colv, i, kb, bad);
   }
-  for (int e = t; e < kb * kb; e += 256) {
-    const int i = e / kb, j = e % kb;
-    X[(k0 + i) * n + k0 + j] = (j <= i) ? a[i][j] : 0.0;
+}
+
+// Column sweep of L x = e_j, steps Q..CB-1: x[Q] *= 1/L[Q][Q], then x[m] -= L[m][Q] x[Q] for m > Q.
+template <int Q>
+__device__ __forceinline__ void inv_steps(double (&x)[CB], const double (*Ls)[CB + 1], const double* rdiag) {
+  if constexpr (Q < CB) {
+    const double xq = x[Q] * rdiag[Q];
+    x[Q] = xq;
+    tri_sweep<Q>(x, xq, [&](int m) { return Ls[m][Q]; });
+    inv_steps<Q + 1>(x, Ls, rdiag);
   }
+}
+
+// One diagonal block of at most 64 x 64 on one wave (the workgroup), in registers.  FACTOR: Cholesky of
+// the block of A at (k0, k0) (lower triangle read; rows >= kb padded with the identity), written back to
+// A with the block's upper triangle zeroed; info receives the first non-positive pivot (1-based, global
+// index).  Without FACTOR the block of A already holds L.  Then X = L^-1 of the block into X at (x0, x0),
+// lane j solving L x = e_j with L broadcast from LDS.
+template <bool FACTOR>
+__global__ void __launch_bounds__(64) tri_leaf_kernel(int64_t lda, int64_t k0, int kb, double* __restrict__ A,
+                                                      double* __restrict__ X, int64_t ldx, int64_t x0,
+                                                      int* __restrict__ info) {
+  __shared__ double Ls[CB][CB + 1];
+  __shared__ double colv[CB];
+  __shared__ double rdiag[CB];
+  const int i = threadIdx.x;
+  const double* arow = A + (k0 + (i < kb ? i : 0)) * lda + k0;  // clamped: every load below is in bounds
+  double r[CB];
+#pragma unroll
+  for (int j = 0; j < CB; ++j) {
+    const double v = arow[j < kb ? j : 0];
+    r[j] = (i < kb && j <= i) ? v : (i == j ? 1.0 : 0.0);
+  }
+  if (FACTOR) {
+    int bad = 0;
+    chol_steps<0>(r, colv, i, kb, bad);
+    if (i == 0 && bad && *info == 0) *info = (int)(k0 + bad);
+  }
+#pragma unroll
+  for (int j = 0; j < CB; ++j) Ls[i][j] = (j <= i) ? r[j] : 0.0;
+  __syncthreads();
+  if (FACTOR) {
+    for (int rr = 0; rr < kb; ++rr)
+      if (i < kb) A[(k0 + rr) * lda + k0 + i] = Ls[rr][i];
+  }
+  rdiag[i] = 1.0 / Ls[i][i];
+  __syncthreads();
+  double x[CB];
+#pragma unroll
+  for (int m = 0; m < CB; ++m) x[m] = (m == i) ? 1.0 : 0.0;
+  inv_steps<0>(x, Ls, rdiag);
+#pragma unroll
+  for (int m = 0; m < CB; ++m)  // unrolled: x stays in registers
+    if (i < kb && m < kb) X[(x0 + m) * ldx + x0 + i] = x[m];
 }
 
 // dst[r][c] = src[r][c] for an rows x cols block with leading dimensions lds / ldd.
@@ -182,14 +179,73 @@ __global__ void transpose_kernel(int64_t n, const double* __restrict__ A, double
     B[j * n + i] = A[e];
   }
 }
+
+// Split of a diagonal range of sz > CB rows at a block boundary near its middle.
+inline int64_t split_rows(int64_t sz) { return ((sz + CB - 1) / CB / 2) * CB; }
+
+// Recursive Cholesky with inverse of the diagonal range [off, off + sz) of A (leading dimension n):
+//   (L11, X11) = chol_inv(A11); L21 = A21 X11^T; A22 -= L21 L21^T; (L22, X22) = chol_inv(A22);
+//   X21 = -X22 (L21 X11).
+// The GEMMs have the full K of each level (not the block width), so they fill the chip; leaves are
+// tri_leaf_kernel<true>.  tmp holds n1 x n2 <= sz^2 / 4 doubles.
+int chol_inv_rec(void* stream, int64_t n, int64_t off, int64_t sz, double* A, double* X, double* tmp, int* info) {
+  hipStream_t st = cc_stream(stream);
+  if (sz <= CB) {
+    tri_leaf_kernel<true><<<1, 64, 0, st>>>(n, off, (int)sz, A, X, n, off, info);
+    CC_LAUNCH_CHECK();
+    return CC_OK;
+  }
+  const int64_t n1 = split_rows(sz), n2 = sz - n1;
+  int rc = chol_inv_rec(stream, n, off, n1, A, X, tmp, info);
+  if (rc) return rc;
+  double* A21 = A + (off + n1) * n + off;
+  double* A22 = A21 + n1;
+  const double* X11 = X + off * n + off;
+  double* X21 = X + (off + n1) * n + off;
+  const double* X22 = X21 + n1;
+  rc = cc_gemm_f64(stream, 1, n2, n1, n1, 1.0, A21, 0, n, 1, X11, 0, 1, n, 0.0, tmp, 0, n1, 1);
+  if (rc) return rc;
+  copy2d_kernel<<<cc_blocks(n2 * n1, 256), 256, 0, st>>>(n2, n1, tmp, n1, A21, n);
+  CC_LAUNCH_CHECK();
+  rc = cc_gemm_f64(stream, 1, n2, n2, n1, -1.0, A21, 0, n, 1, A21, 0, 1, n, 1.0, A22, 0, n, 1);
+  if (rc) return rc;
+  rc = chol_inv_rec(stream, n, off + n1, n2, A, X, tmp, info);
+  if (rc) return rc;
+  rc = cc_gemm_f64(stream, 1, n2, n1, n1, 1.0, A21, 0, n, 1, X11, 0, n, 1, 0.0, tmp, 0, n1, 1);
+  if (rc) return rc;
+  return cc_gemm_f64(stream, 1, n2, n1, n2, -1.0, X22, 0, n, 1, tmp, 0, n1, 1, 0.0, X21, 0, n, 1);
+}
+
+// Recursive inverse of the lower-triangular range [off, off + sz) of L: X11, X22 recursively, then
+// X21 = -X22 (L21 X11); leaves are tri_leaf_kernel<false>.
+int trtri_rec(void* stream, int64_t n, int64_t off, int64_t sz, const double* L, double* X, double* tmp) {
+  hipStream_t st = cc_stream(stream);
+  if (sz <= CB) {
+    tri_leaf_kernel<false><<<1, 64, 0, st>>>(n, off, (int)sz, const_cast<double*>(L), X, n, off, nullptr);
+    CC_LAUNCH_CHECK();
+    return CC_OK;
+  }
+  const int64_t n1 = split_rows(sz), n2 = sz - n1;
+  int rc = trtri_rec(stream, n, off, n1, L, X, tmp);
+  if (rc) return rc;
+  rc = trtri_rec(stream, n, off + n1, n2, L, X, tmp);
+  if (rc) return rc;
+  const double* L21 = L + (off + n1) * n + off;
+  const double* X11 = X + off * n + off;
+  double* X21 = X + (off + n1) * n + off;
+  rc = cc_gemm_f64(stream, 1, n2, n1, n1, 1.0, L21, 0, n, 1, X11, 0, n, 1, 0.0, tmp, 0, n1, 1);
+  if (rc) return rc;
+  return cc_gemm_f64(stream, 1, n2, n1, n2, -1.0, X21 + n1, 0, n, 1, tmp, 0, n1, 1, 0.0, X21, 0, n, 1);
+}
 }  // namespace
 
 extern "C" int cc_linalg_work_size(int64_t n, int64_t* out) {
-  *out = (int64_t)CB * CB + (int64_t)CB * n;
+  const int64_t blocked = (int64_t)CB * CB + (int64_t)CB * n, recursive = n * n / 4 + 1;
+  *out = blocked > recursive ? blocked : recursive;
   return CC_OK;
 }
 
-// Right-looking blocked Cholesky: diagonal block in LDS (with its inverse), the panel as a GEMM
+// Right-looking blocked Cholesky: diagonal block factored and inverted on one wave, the panel as a GEMM
 // against that inverse, the trailing update as a GEMM.
 extern "C" int cc_cholesky_f64(void* stream, int64_t n, double* A, int* info_dev, double* work) {
   if (n <= 0) { cc_set_last_error("cholesky: n must be positive"); return CC_ERR_ARG; }
@@ -200,7 +256,7 @@ extern "C" int cc_cholesky_f64(void* stream, int64_t n, double* A, int* info_dev
   CC_LAUNCH_CHECK();
   for (int64_t k0 = 0; k0 < n; k0 += CB) {
     int kb = (int)((n - k0) < CB ? (n - k0) : CB);
-    chol_diag_kernel<<<1, 256, 0, st>>>(n, k0, kb, A, dinv, info_dev);
+    tri_leaf_kernel<true><<<1, 64, 0, st>>>(n, k0, kb, A, dinv, kb, 0, info_dev);
     CC_LAUNCH_CHECK();
     int64_t rest = n - k0 - kb;
     if (rest > 0) {
@@ -220,24 +276,27 @@ extern "C" int cc_cholesky_f64(void* stream, int64_t n, double* A, int* info_dev
   return CC_OK;
 }
 
-// X = L^-1, blocked: diagonal-block inverses, then block row i: X[i, :i] = -X_ii (L[i, :i] X[:i, :i]).
-extern "C" int cc_trtri_lower_f64(void* stream, int64_t n, const double* L, double* X, double* work) {
-  if (n <= 0) { cc_set_last_error("trtri: n must be positive"); return CC_ERR_ARG; }
+// A = L L^T in place (upper zeroed) and X = L^-1 (upper zero) in one recursive pass.
+extern "C" int cc_cholesky_inverse_f64(void* stream, int64_t n, double* A, double* X, int* info_dev, double* work) {
+  if (n <= 0) { cc_set_last_error("cholesky_inverse: n must be positive"); return CC_ERR_ARG; }
   hipStream_t st = cc_stream(stream);
+  set_int_kernel<<<1, 1, 0, st>>>(info_dev, 0);
+  CC_LAUNCH_CHECK();
   int rc = cc_fill_f64(stream, n * n, 0.0, X);
   if (rc) return rc;
-  const int64_t nblk = (n + CB - 1) / CB;
-  diag_block_inverse_kernel<<<(unsigned)nblk, 256, 0, st>>>(n, L, X);
+  rc = chol_inv_rec(stream, n, 0, n, A, X, work, info_dev);
+  if (rc) return rc;
+  zero_upper_kernel<<<cc_blocks(n * n, 256), 256, 0, st>>>(n, A);
   CC_LAUNCH_CHECK();
-  double* S = work;  // CB x n
-  for (int64_t i = 1; i < nblk; ++i) {
-    int64_t r0 = i * CB, rb = (n - r0) < CB ? (n - r0) : CB, c = r0;
-    rc = cc_gemm_f64(stream, 1, rb, c, c, 1.0, L + r0 * n, 0, n, 1, X, 0, n, 1, 0.0, S, 0, c, 1);
-    if (rc) return rc;
-    rc = cc_gemm_f64(stream, 1, rb, c, rb, -1.0, X + r0 * n + r0, 0, n, 1, S, 0, c, 1, 0.0, X + r0 * n, 0, n, 1);
-    if (rc) return rc;
-  }
   return CC_OK;
+}
+
+// X = L^-1 for lower-triangular L, recursively (leaves on one wave each, GEMMs for the off-diagonal blocks).
+extern "C" int cc_trtri_lower_f64(void* stream, int64_t n, const double* L, double* X, double* work) {
+  if (n <= 0) { cc_set_last_error("trtri: n must be positive"); return CC_ERR_ARG; }
+  int rc = cc_fill_f64(stream, n * n, 0.0, X);
+  if (rc) return rc;
+  return trtri_rec(stream, n, 0, n, L, X, work);
 }
 
 extern "C" int cc_transpose_f64(void* stream, int64_t n, const double* A, double* B) {

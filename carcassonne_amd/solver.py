@@ -46,14 +46,13 @@ class NormalizationFactor:
         L = Nenv.clone()
         info = torch.zeros(1, dtype=torch.int32, device=L.device)
         work = empty((_lib.query("cc_linalg_work_size", self.n_env),))
-        _lib.call("cc_cholesky_f64", stream(), self.n_env, ptr(L), ptr(info), ptr(work))
+        self.W = empty(L.shape)
+        _lib.call("cc_cholesky_inverse_f64", stream(), self.n_env, ptr(L), ptr(self.W), ptr(info), ptr(work))
         if int(info.item()) != 0:
             raise ValueError("the normalization matrix is not positive definite (pivot {})".format(int(info.item())))
         self.L = L
         self.Lt = empty(L.shape)
         _lib.call("cc_transpose_f64", stream(), self.n_env, ptr(L), ptr(self.Lt))
-        self.W = empty(L.shape)
-        _lib.call("cc_trtri_lower_f64", stream(), self.n_env, ptr(L), ptr(self.W), ptr(work))
         self.Wt = empty(L.shape)
         _lib.call("cc_transpose_f64", stream(), self.n_env, ptr(self.W), ptr(self.Wt))
 
@@ -124,18 +123,17 @@ class _DenseInverse:
         trace = float(torch.diagonal(base).sum().item())
         work = empty((_lib.query("cc_linalg_work_size", n),))
         info = torch.zeros(1, dtype=torch.int32, device=base.device)
+        self.W = empty((n, n))
         for shift in (0.0, 1e-14, 1e-12, 1e-10):
             L = base.clone()
             if shift:
                 L.view(-1)[:: n + 1] += shift * abs(trace) / n
-            _lib.call("cc_cholesky_f64", stream(), n, ptr(L), ptr(info), ptr(work))
+            _lib.call("cc_cholesky_inverse_f64", stream(), n, ptr(L), ptr(self.W), ptr(info), ptr(work))
             if int(info.item()) == 0:
                 break
         else:
             raise ValueError("the normalization matrix is not positive definite (pivot {}); the HIP path "
                              "factors N by Cholesky".format(int(info.item())))
-        self.W = empty(L.shape)
-        _lib.call("cc_trtri_lower_f64", stream(), n, ptr(L), ptr(self.W), ptr(work))
         self.Wt = empty(L.shape)
         _lib.call("cc_transpose_f64", stream(), n, ptr(self.W), ptr(self.Wt))
         self.n = n

@@ -154,6 +154,10 @@ int cc_linalg_work_size(int64_t n, int64_t* out);
 int cc_cholesky_f64(void* stream, int64_t n, double* A, int* info_dev, double* work);
 /* X = L^-1 for lower-triangular L (n x n); `work` holds cc_linalg_work_size doubles. */
 int cc_trtri_lower_f64(void* stream, int64_t n, const double* L, double* X, double* work);
+/* A = L L^T in place (upper zeroed) and X = L^-1 in one recursive pass; info_dev as cc_cholesky_f64;
+ * `work` holds cc_linalg_work_size doubles.  The normalization factor of the whitened eigenproblem
+ * (the reference inverts N through relaxOver's LU, utils.py:814-817). */
+int cc_cholesky_inverse_f64(void* stream, int64_t n, double* A, double* X, int* info_dev, double* work);
 /* B = A^T (n x n). */
 int cc_transpose_f64(void* stream, int64_t n, const double* A, double* B);
 /* Y (n x nrhs) = M (n x n) X restricted to tri = 0 full, 1 lower, 2 upper; nrhs <= 4. */

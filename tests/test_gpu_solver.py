@@ -39,6 +39,46 @@ def test_cholesky_trtri_skinny(gpu):
     assert int(info.item()) == 1
 
 
+def test_cholesky_inverse_recursive(gpu):
+    """cc_cholesky_inverse_f64: L equals numpy's Cholesky factor, X = L^-1 (X L = I), both lower with a
+    zero upper triangle, for sizes that are and are not multiples of the 64-row leaves; a non-positive
+    pivot inside a later leaf is reported with its 1-based global index by both Cholesky entry points."""
+    from carcassonne_amd import _lib
+    from carcassonne_amd.device import ptr, stream, empty
+    rng = np.random.default_rng(11)
+    for n in (1, 7, 64, 65, 128, 191, 333, 1296):
+        g = rng.standard_normal((n, n))
+        a = g @ g.T + n * np.eye(n)
+        A = torch.from_numpy(a.copy()).cuda()
+        X = empty((n, n))
+        info = torch.full((1,), 5, dtype=torch.int32, device=A.device)
+        work = empty((_lib.query("cc_linalg_work_size", n),))
+        _lib.call("cc_cholesky_inverse_f64", stream(), n, ptr(A), ptr(X), ptr(info), ptr(work))
+        assert int(info.item()) == 0
+        L, Xh = A.cpu().numpy(), X.cpu().numpy()
+        want = np.linalg.cholesky(a)
+        np.testing.assert_allclose(L, want, rtol=1e-11, atol=1e-11)
+        assert np.all(np.triu(L, 1) == 0.0) and np.all(np.triu(Xh, 1) == 0.0)
+        np.testing.assert_allclose(Xh @ want, np.eye(n), atol=1e-11)
+        # the recursive trtri on the same factor agrees
+        X2 = empty((n, n))
+        _lib.call("cc_trtri_lower_f64", stream(), n, ptr(A), ptr(X2), ptr(work))
+        np.testing.assert_allclose(X2.cpu().numpy(), Xh, rtol=1e-12, atol=1e-12)
+    for n, bad in ((200, 131), (333, 64), (100, 100)):
+        a = np.eye(n)
+        a[bad - 1, bad - 1] = -1.0
+        for name in ("cc_cholesky_inverse_f64", "cc_cholesky_f64"):
+            A = torch.from_numpy(a.copy()).cuda()
+            info = torch.zeros(1, dtype=torch.int32, device=A.device)
+            work = empty((_lib.query("cc_linalg_work_size", n),))
+            if name == "cc_cholesky_f64":
+                _lib.call(name, stream(), n, ptr(A), ptr(info), ptr(work))
+            else:
+                X = empty((n, n))
+                _lib.call(name, stream(), n, ptr(A), ptr(X), ptr(info), ptr(work))
+            assert int(info.item()) == bad, (name, n, bad, int(info.item()))
+
+
 @pytest.mark.parametrize("D,chi,model,seed", [(2, 8, "ising", 0), (3, 6, "heisenberg", 1), (4, 8, "ising", 2),
                                               (4, 16, "heisenberg", 3)])
 def test_lanczos_matches_generalized_eigh(gpu, D, chi, model, seed):
