@@ -41,6 +41,7 @@ SIGNATURES = {
     "cc_stage3_packed_size": [_i64, _i64, _pi64],
     "cc_stage3_fused_work_size": [_i64, _i64, _i64, _pi64],
     "cc_stage3_pack_f64": [_vp, _i64, _i64, _vp, _vp],
+    "cc_stage3_pack_strided_f64": [_vp, _i64, _i64, _i64, _i64, _vp, _vp, _vp],
     "cc_stage3_plan_create": [ctypes.POINTER(ctypes.c_void_p), _i32, _pvp, _i32, _pvp, _i32, _pi32, _pi32, _pi32,
                               _i32, _pf64, _i64, _i64, _i64, _vp],
     "cc_stage3_plan_apply": [_vp, _vp, _vp, _vp],
@@ -57,6 +58,7 @@ SIGNATURES = {
     "cc_lanczos_f64": [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _f64, _f64, _i32, _i32, _vp, _vp, _pf64],
     "cc_stage3_lp_slab_size": [_i64, _i64, _pi64],
     "cc_stage3_lp_convert": [_vp, _i32, _i64, _i64, _vp, _vp, _vp],
+    "cc_stage3_lp_convert_strided": [_vp, _i32, _i64, _i64, _i64, _i64, _vp, _i64, _vp, _vp, _vp],
     "cc_stage3_lp_plan_chunks": [_vp, _pi32],
     "cc_stage3_lp_work_size": [_i64, _i64, _i64, _i32, _pi64],
     "cc_stage3_lp_plan_create": [ctypes.POINTER(ctypes.c_void_p), _i32, _i32, _pvp, _i32, _pvp, _i32, _pi32, _pi32,

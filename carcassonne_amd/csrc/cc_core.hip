@@ -10,6 +10,20 @@ static thread_local std::string g_last_error;
 void cc_set_last_error(const char* msg) { g_last_error = msg ? msg : ""; }
 
 extern "C" int cc_version(void) { return 1; }
+
+int cc_slab_map(int64_t na0, int64_t na1, int64_t d0, int64_t d1, const int64_t* strides, CCSlabMap* out) {
+  if (na0 < 1 || na1 < 1 || d0 < 1 || d1 < 1 || strides == nullptr) {
+    cc_set_last_error("slab map: leg sizes must be positive");
+    return CC_ERR_ARG;
+  }
+  out->na1 = na1;
+  out->d1 = d1;
+  for (int i = 0; i < 6; ++i) {
+    if (strides[i] < 0) { cc_set_last_error("slab map: strides must be non-negative"); return CC_ERR_ARG; }
+    out->s[i] = strides[i];
+  }
+  return CC_OK;
+}
 extern "C" const char* cc_last_error(void) { return g_last_error.c_str(); }
 
 // ---------------------------------------------------------------- permute

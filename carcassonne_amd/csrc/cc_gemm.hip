@@ -1,11 +1,13 @@
 // FP64 GEMM on CDNA4 matrix cores (v_mfma_f64_16x16x4f64) with arbitrary element
-// strides, batched over grid.z.  Tile 64x64x16 per 256-thread workgroup; each wave
-// owns a 32x32 sub-tile (2x2 MFMA tiles).  Operand fragments (probed on gfx950):
+// strides, batched over grid.z.  Tile 64x64x32 per 256-thread workgroup; each wave
+// owns a 32x32 sub-tile (2x2 MFMA tiles), 32 MFMAs per K step, which covers the
+// global-load latency of the next step's operands (prefetched into registers).
+// Operand fragments (probed on gfx950):
 //   A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane j + 16(i%4), reg i/4.
 #include "cc_common.h"
 
 namespace {
-constexpr int BM = 64, BN = 64, BK = 16, PAD = 2;
+constexpr int BM = 64, BN = 64, BK = 32, PAD = 2;
 
 struct GemmArgs {
   int64_t M, N, K;
@@ -33,38 +35,39 @@ __global__ void __launch_bounds__(256) gemm_f64_kernel(GemmArgs g) {
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b) acc[a][b] = (f64x4){0.0, 0.0, 0.0, 0.0};
 
-  double ra[4], rb[4];
+  constexpr int NL = BM * BK / 256;  // operand elements per thread and matrix per K step
+  double ra[NL], rb[NL];
   auto load = [&](int64_t k0) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < NL; ++r) {
       int i, k;
-      if (A_K_FAST) { k = tid & 15; i = (tid >> 4) + 16 * r; }
+      if (A_K_FAST) { k = tid & (BK - 1); i = tid / BK + (256 / BK) * r; }
       else { i = tid & 63; k = (tid >> 6) + 4 * r; }
       int64_t gi = m0 + i, gk = k0 + k;
       ra[r] = (gi < g.M && gk < g.K) ? A[gi * g.sAm + gk * g.sAk] : 0.0;
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < NL; ++r) {
       int j, k;
       if (B_N_FAST) { j = tid & 63; k = (tid >> 6) + 4 * r; }
-      else { k = tid & 15; j = (tid >> 4) + 16 * r; }
+      else { k = tid & (BK - 1); j = tid / BK + (256 / BK) * r; }
       int64_t gj = n0 + j, gk = k0 + k;
       rb[r] = (gj < g.N && gk < g.K) ? B[gk * g.sBk + gj * g.sBn] : 0.0;
     }
   };
   auto store = [&]() {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < NL; ++r) {
       int i, k;
-      if (A_K_FAST) { k = tid & 15; i = (tid >> 4) + 16 * r; }
+      if (A_K_FAST) { k = tid & (BK - 1); i = tid / BK + (256 / BK) * r; }
       else { i = tid & 63; k = (tid >> 6) + 4 * r; }
       As[k][i] = ra[r];
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < NL; ++r) {
       int j, k;
       if (B_N_FAST) { j = tid & 63; k = (tid >> 6) + 4 * r; }
-      else { k = tid & 15; j = (tid >> 4) + 16 * r; }
+      else { k = tid & (BK - 1); j = tid / BK + (256 / BK) * r; }
       Bs[k][j] = rb[r];
     }
   };

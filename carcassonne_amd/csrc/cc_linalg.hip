@@ -14,7 +14,7 @@ __device__ inline double readlane_f64(double v, int l) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-constexpr int CH = 8, NCH = CB / CH;  // broadcast operands are read from LDS in chunks of CH
+constexpr int CH = 16, NCH = CB / CH;  // broadcast operands are read from LDS in chunks of CH
 
 // r[j] -= s * src(j) for every j > K.  src is read in chunks of CH, one chunk ahead of its FMAs; the
 // address of each chunk's loads depends (through an empty asm that takes the FMA results) on the chunk
@@ -36,8 +36,11 @@ __device__ __forceinline__ void tri_sweep(double (&r)[CB], double s, F src) {
 #pragma unroll
       for (int u = 0; u < CH; ++u)
         if (c * CH + u > K) r[c * CH + u] -= s * cur[u];
-      __asm__ volatile("" : "+v"(tok) : "v"(r[c * CH + 0]), "v"(r[c * CH + 1]), "v"(r[c * CH + 2]), "v"(r[c * CH + 3]),
-                       "v"(r[c * CH + 4]), "v"(r[c * CH + 5]), "v"(r[c * CH + 6]), "v"(r[c * CH + 7]));
+#pragma unroll
+      for (int h = 0; h < CH; h += 8)
+        __asm__ volatile("" : "+v"(tok) : "v"(r[c * CH + h + 0]), "v"(r[c * CH + h + 1]), "v"(r[c * CH + h + 2]),
+                         "v"(r[c * CH + h + 3]), "v"(r[c * CH + h + 4]), "v"(r[c * CH + h + 5]), "v"(r[c * CH + h + 6]),
+                         "v"(r[c * CH + h + 7]));
 #pragma unroll
       for (int u = 0; u < CH; ++u) cur[u] = nxt[u];
     }

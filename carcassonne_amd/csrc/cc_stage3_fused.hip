@@ -40,15 +40,24 @@ struct Shape {
   static constexpr int SLAB = KS * NT * 64;        // packed doubles per alpha
 };
 
-__global__ void pack_kernel(int64_t n_alpha, int P, int NT, int KS, const double* __restrict__ src, double* __restrict__ dst) {
-  const int64_t per = (int64_t)KS * NT * 64;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n_alpha * per; e += (int64_t)gridDim.x * blockDim.x) {
-    int64_t a = e / per;
-    int r = (int)(e % per);
-    int lane = r & 63, t = r >> 6;
-    int n = t % NT, u = t / NT;
-    int row = 16 * n + (lane & 15), col = 4 * u + (lane >> 4);
-    dst[e] = (row < P && col < P) ? src[(a * P + row) * P + col] : 0.0;
+constexpr int PACK_MAX_P = 64;
+// One alpha per workgroup step: the slab is gathered into LDS in source order (coalesced reads), then
+// written in MFMA-operand order (packed[alpha][u][n][lane] = S[16n + lane%16][4u + lane/16], zero padded).
+__global__ void __launch_bounds__(256) pack_kernel(int64_t n_alpha, int NT, int KS, CCSlabGather g,
+                                                   const double* __restrict__ src, double* __restrict__ dst) {
+  extern __shared__ double s[];  // P x (P + 1)
+  const int P = g.P, ld = P + 1;
+  const int per = KS * NT * 64;
+  for (int64_t al = blockIdx.x; al < n_alpha; al += gridDim.x) {
+    cc_slab_load(g, src, al, s, ld, 1.0);
+    __syncthreads();
+    double* out = dst + al * per;
+    for (int r = threadIdx.x; r < per; r += blockDim.x) {
+      const int lane = r & 63, t = r >> 6, n = t % NT, u = t / NT;
+      const int row = 16 * n + (lane & 15), col = 4 * u + (lane >> 4);
+      out[r] = (row < P && col < P) ? s[row * ld + col] : 0.0;
+    }
+    __syncthreads();
   }
 }
 
@@ -279,14 +288,26 @@ extern "C" int cc_stage3_fused_work_size(int64_t n_alpha, int64_t P, int64_t d, 
   return CC_OK;
 }
 
-extern "C" int cc_stage3_pack_f64(void* stream, int64_t n_alpha, int64_t P, const double* slabs, double* packed) {
-  if (n_alpha < 0 || P <= 0) { cc_set_last_error("pack: bad sizes"); return CC_ERR_ARG; }
+extern "C" int cc_stage3_pack_strided_f64(void* stream, int64_t na0, int64_t na1, int64_t d0, int64_t d1,
+                                          const int64_t* strides, const double* src, double* packed) {
+  CCSlabMap map;
+  int rc = cc_slab_map(na0, na1, d0, d1, strides, &map);
+  if (rc) return rc;
+  const int64_t n_alpha = na0 * na1, P = d0 * d1;
+  if (P > PACK_MAX_P) { cc_set_last_error("pack: slab extent above 64"); return CC_ERR_UNSUPPORTED; }
   int NT = (int)((P + 15) / 16), KS = (int)((P + 3) / 4);
-  int64_t total = n_alpha * KS * NT * 64;
-  if (total == 0) return CC_OK;
-  pack_kernel<<<cc_blocks(total, 256), 256, 0, cc_stream(stream)>>>(n_alpha, (int)P, NT, KS, slabs, packed);
+  const unsigned grid = (unsigned)(n_alpha < 16384 ? n_alpha : 16384);
+  const size_t lds = (size_t)P * (P + 1) * sizeof(double);
+  pack_kernel<<<grid, 256, lds, cc_stream(stream)>>>(n_alpha, NT, KS, cc_slab_gather(map, d0), src, packed);
   CC_LAUNCH_CHECK();
   return CC_OK;
+}
+
+extern "C" int cc_stage3_pack_f64(void* stream, int64_t n_alpha, int64_t P, const double* slabs, double* packed) {
+  if (n_alpha < 0 || P <= 0) { cc_set_last_error("pack: bad sizes"); return CC_ERR_ARG; }
+  if (n_alpha == 0) return CC_OK;
+  const int64_t st[6] = {P * P, 0, P, 0, 1, 0};
+  return cc_stage3_pack_strided_f64(stream, n_alpha, 1, P, 1, st, slabs, packed);
 }
 
 extern "C" int cc_stage3_matvec_fused_f64(void* stream, int n_a, const double* const* A_packed, int n_b,

@@ -369,10 +369,39 @@ __global__ void __launch_bounds__(LP_RCOLS * LP_RGROUPS) lp_reduce_kernel(int64_
 
 
 // low-precision slab = slab / max|slab|: pass 1 per-block maxima, pass 2 (one block) their maximum
+// max |x| over n contiguous elements: block partials (four independent loads in flight per thread),
+// then one block
 __global__ void __launch_bounds__(256) lp_blockmax_kernel(int64_t n, const double* __restrict__ x, double* __restrict__ part) {
   __shared__ double red[256];
+  double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    m0 = fmax(m0, fabs(x[i]));
+    m1 = fmax(m1, fabs(x[i + stride]));
+    m2 = fmax(m2, fabs(x[i + 2 * stride]));
+    m3 = fmax(m3, fabs(x[i + 3 * stride]));
+  }
+  for (; i < n; i += stride) m0 = fmax(m0, fabs(x[i]));
+  double m = fmax(fmax(m0, m1), fmax(m2, m3));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+// the same over a channel read through the slab map (a map that does not cover its source exactly)
+__global__ void __launch_bounds__(256) lp_blockmax_map_kernel(int64_t n_alpha, int64_t P, CCSlabMap map,
+                                                              const double* __restrict__ x, double* __restrict__ part) {
+  __shared__ double red[256];
+  const int64_t PP = P * P, n = n_alpha * PP;
   double m = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) m = fmax(m, fabs(x[i]));
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const int64_t al = i / PP, e = i - al * PP;
+    m = fmax(m, fabs(x[cc_slab_offset(map, al, e / P, e % P)]));
+  }
   red[threadIdx.x] = m;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
@@ -393,15 +422,22 @@ __global__ void __launch_bounds__(1024) lp_absmax_kernel(int64_t n, const double
   }
   if (threadIdx.x == 0) out[0] = red[0] > 0.0 ? red[0] : 1.0;
 }
-// y[alpha][i] = x[alpha][i] / scale for i < P*P, zero in the stride padding
+constexpr int LP_CONVERT_MAX_P = 64;  // dynamic LDS of P x (P + 1) doubles
+// y[alpha][i] = slab[alpha][i / P][i % P] / scale for i < P*P, zero in the stride padding; each slab is
+// gathered into LDS in source order first (coalesced reads), then written row-major
 template <typename T>
-__global__ void lp_convert_kernel(int64_t n_alpha, int64_t PP, int64_t SP, const double* __restrict__ x,
-                                  const double* __restrict__ scale, T* __restrict__ y) {
+__global__ void __launch_bounds__(256) lp_convert_kernel(int64_t n_alpha, int64_t SP, CCSlabGather g,
+                                                         const double* __restrict__ x, const double* __restrict__ scale,
+                                                         T* __restrict__ y) {
+  extern __shared__ double s[];  // P x (P + 1)
   const double inv = 1.0 / scale[0];
-  const int64_t total = n_alpha * SP;
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t al = k / SP, i = k - al * SP;
-    y[k] = i < PP ? (T)(x[al * PP + i] * inv) : (T)0.0f;
+  const int P = g.P, ld = P + 1, PP = P * P;
+  for (int64_t al = blockIdx.x; al < n_alpha; al += gridDim.x) {
+    cc_slab_load(g, x, al, s, ld, inv);
+    __syncthreads();
+    T* out = y + al * SP;
+    for (int i = threadIdx.x; i < SP; i += blockDim.x) out[i] = i < PP ? (T)s[(i / P) * ld + i % P] : (T)0.0f;
+    __syncthreads();
   }
 }
 
@@ -561,21 +597,37 @@ extern "C" int cc_stage3_lp_slab_size(int64_t n_alpha, int64_t P, int64_t* out) 
   return CC_OK;
 }
 
-extern "C" int cc_stage3_lp_convert(void* stream, int dtype, int64_t n_alpha, int64_t P, const double* slab,
-                                    double* scale_out, void* out) {
+extern "C" int cc_stage3_lp_convert_strided(void* stream, int dtype, int64_t na0, int64_t na1, int64_t d0, int64_t d1,
+                                            const int64_t* strides, int64_t src_elems, const double* src,
+                                            double* scale_out, void* out) {
+  CCSlabMap map;
+  int rc = cc_slab_map(na0, na1, d0, d1, strides, &map);
+  if (rc) return rc;
+  if (!(dtype == CC_F16 || dtype == CC_BF16)) { cc_set_last_error("lp convert: dtype must be f16 or bf16"); return CC_ERR_ARG; }
   hipStream_t st = cc_stream(stream);
-  if (n_alpha <= 0) return CC_OK;
-  const int64_t n = n_alpha * P * P, SP = lp_stride((int)P);
-  lp_blockmax_kernel<<<512, 256, 0, st>>>(n, slab, scale_out + 1);   // scale_out holds 513 doubles
+  const int64_t n_alpha = na0 * na1, P = d0 * d1, SP = lp_stride((int)P);
+  if (P > LP_CONVERT_MAX_P) { cc_set_last_error("lp convert: slab extent above 64"); return CC_ERR_UNSUPPORTED; }
+  // scale_out holds 513 doubles: block maxima in [1, 513), the maximum in [0]
+  if (src_elems == n_alpha * P * P) lp_blockmax_kernel<<<512, 256, 0, st>>>(src_elems, src, scale_out + 1);
+  else lp_blockmax_map_kernel<<<512, 256, 0, st>>>(n_alpha, P, map, src, scale_out + 1);
   CC_LAUNCH_CHECK();
   lp_absmax_kernel<<<1, 1024, 0, st>>>(512, scale_out + 1, scale_out);
   CC_LAUNCH_CHECK();
-  const unsigned blocks = cc_blocks(n_alpha * SP, 256);
-  if (dtype == CC_F16) lp_convert_kernel<_Float16><<<blocks, 256, 0, st>>>(n_alpha, P * P, SP, slab, scale_out, (_Float16*)out);
-  else if (dtype == CC_BF16) lp_convert_kernel<__bf16><<<blocks, 256, 0, st>>>(n_alpha, P * P, SP, slab, scale_out, (__bf16*)out);
-  else { cc_set_last_error("lp convert: dtype must be f16 or bf16"); return CC_ERR_ARG; }
+  const unsigned grid = (unsigned)(n_alpha < 16384 ? n_alpha : 16384);
+  const CCSlabGather g = cc_slab_gather(map, d0);
+  const size_t lds = (size_t)P * (P + 1) * sizeof(double);
+  if (dtype == CC_F16) lp_convert_kernel<_Float16><<<grid, 256, lds, st>>>(n_alpha, SP, g, src, scale_out, (_Float16*)out);
+  else lp_convert_kernel<__bf16><<<grid, 256, lds, st>>>(n_alpha, SP, g, src, scale_out, (__bf16*)out);
   CC_LAUNCH_CHECK();
   return CC_OK;
+}
+
+extern "C" int cc_stage3_lp_convert(void* stream, int dtype, int64_t n_alpha, int64_t P, const double* slab,
+                                    double* scale_out, void* out) {
+  if (n_alpha <= 0) return CC_OK;
+  if (P < 1) { cc_set_last_error("lp convert: bad sizes"); return CC_ERR_ARG; }
+  const int64_t st[6] = {P * P, 0, P, 0, 1, 0};
+  return cc_stage3_lp_convert_strided(stream, dtype, n_alpha, 1, P, 1, st, n_alpha * P * P, slab, scale_out, out);
 }
 
 extern "C" int cc_stage3_lp_work_size(int64_t n_alpha, int64_t P, int64_t d, int groups, int64_t* out) {

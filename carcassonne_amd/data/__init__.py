@@ -271,6 +271,17 @@ class HipData(Data):
                       _lib.i64_array(shape), _lib.i64_array([st[p] for p in perm]))
         return out
 
+    def strided_source(self):
+        """(contiguous device tensor, element strides of this tensor's legs in it)."""
+        return self._t, _strides(self.shape)
+
+    def permuted_lazily(self, perm):
+        """This tensor with its legs permuted, materialised only when a kernel needs the permuted data."""
+        perm = [int(p) for p in perm]
+        if perm == list(range(self.ndim)):
+            return self
+        return PermutedView(self._t, perm)
+
     def transpose(self, *args):
         if len(args) == 1 and not isinstance(args[0], (int, np.integer)):
             args = tuple(args[0])
@@ -470,3 +481,70 @@ HipData.X = _DeviceConstant([[0, 1], [1, 0]], "X")
 HipData.Z = _DeviceConstant([[1, 0], [0, -1]], "Z")
 HipData.iY = _DeviceConstant([[0, 1], [-1, 0]], "iY")
 HipData.Y = _DeviceConstant(None, "Y")
+
+
+class PermutedView(HipData):
+    """A leg permutation of a contiguous device tensor that is carried out (cc_permute) only when some
+    kernel needs the permuted data as a contiguous array.  Readers that take strides (the stage-3 slab
+    packers) use strided_source() and skip the copy.  Constructed with a single tensor it is a plain
+    HipData (so classmethods called on type(x) keep working)."""
+
+    def __init__(self, base, perm=None):
+        if perm is None:
+            HipData.__init__(self, base)
+            return
+        if not (isinstance(base, torch.Tensor) and base.is_contiguous() and base.dtype == torch.float64):
+            raise TypeError("PermutedView needs a contiguous float64 device tensor")
+        self._base, self._perm, self._mat = base, [int(p) for p in perm], None
+
+    @property
+    def _t(self):
+        if self._mat is None:
+            self._mat = HipData(self._base)._permuted(self._perm)
+            self._base = None
+        return self._mat
+
+    @_t.setter
+    def _t(self, value):
+        self._mat, self._base = value, None
+
+    def _lazy(self):
+        return self.__dict__.get("_mat") is None and self.__dict__.get("_base") is not None
+
+    @property
+    def shape(self):
+        if self._lazy():
+            return tuple(int(self._base.shape[p]) for p in self._perm)
+        return tuple(self._t.shape)
+
+    @property
+    def ndim(self):
+        return len(self.shape)
+
+    def size(self):
+        return _prod(self.shape)
+
+    def strided_source(self):
+        if self._lazy():
+            st = _strides(tuple(self._base.shape))
+            return self._base, [st[p] for p in self._perm]
+        return self._t, _strides(self.shape)
+
+    def _same_lazy_layout(self, other):
+        return (self._lazy() and isinstance(other, PermutedView) and other._lazy() and other._perm == self._perm and
+                tuple(other._base.shape) == tuple(self._base.shape))
+
+    def __iadd__(self, other):
+        # sums of contraction results with the same leg permutation stay unpermuted (sparse accumulation)
+        if self._same_lazy_layout(other):
+            _lib.call("cc_axpby_f64", stream(), self._base.numel(), 1.0, ptr(other._base), 1.0, ptr(self._base))
+            return self
+        return HipData.__iadd__(self, other)
+
+    def __add__(self, other):
+        if self._same_lazy_layout(other):
+            out = empty(tuple(self._base.shape))
+            _lib.call("cc_binary_f64", stream(), 2, out.numel(), ptr(self._base), ptr(other._base), ptr(out))
+            return PermutedView(out, self._perm)
+        return HipData.__add__(self, other)
+

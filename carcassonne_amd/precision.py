@@ -43,14 +43,24 @@ class LowPrecisionOperator:
                 b_list.append(B)
             rows.append((a_ids[id(A)], k, b_ids[id(B)], O))
         rows.sort(key=lambda r: (r[0], r[1]))
-        self._slabs, scales = [], []
-        sc = empty((len(a_list) + len(b_list), 513))
-        slab_elems = _lib.query("cc_stage3_lp_slab_size", self.n_alpha, self.P)
-        for i, ch in enumerate(a_list + b_list):
-            buf = torch.empty(slab_elems, dtype=torch.float16 if dtype == "f16" else torch.bfloat16, device=sc.device)
-            _lib.call("cc_stage3_lp_convert", stream(), code, self.n_alpha, self.P, ptr(ch._t), ptr(sc[i]), ptr(buf))
-            self._slabs.append(buf)
-        scales = sc[:, 0].tolist()
+        # each channel is converted once (read in place from its stage-2 tensor) and kept on the channel,
+        # so operators built from the same stage-2 tensors share the rounded slabs
+        fresh = [ch for ch in a_list + b_list if dtype not in ch.lowprec]
+        if fresh:
+            sc = empty((len(fresh), 513))
+            slab_elems = _lib.query("cc_stage3_lp_slab_size", self.n_alpha, self.P)
+            bufs = []
+            for i, ch in enumerate(fresh):
+                buf = torch.empty(slab_elems, dtype=torch.float16 if dtype == "f16" else torch.bfloat16, device=sc.device)
+                base, (na0, na1, d0, d1), st = ch.source()
+                exact = base.numel() == na0 * na1 * d0 * d1 * d0 * d1   # the map covers the stage-2 tensor
+                _lib.call("cc_stage3_lp_convert_strided", stream(), code, na0, na1, d0, d1, _lib.i64_array(st),
+                          base.numel() if exact else 0, ptr(base), ptr(sc[i]), ptr(buf))
+                bufs.append(buf)
+            for ch, buf, scale in zip(fresh, bufs, sc[:, 0].tolist()):
+                ch.lowprec[dtype] = (buf, scale)
+        self._slabs = [ch.lowprec[dtype][0] for ch in a_list + b_list]
+        scales = [ch.lowprec[dtype][1] for ch in a_list + b_list]
         sa, sb = scales[:len(a_list)], scales[len(a_list):]
         coef = []
         for (a, _, b, O) in rows:
@@ -58,7 +68,7 @@ class LowPrecisionOperator:
             coef.extend((sa[a] * sb[b] * m).ravel().tolist())
         self.groups = int(groups if groups is not None else max(1, min(self.n_alpha // 8, 2048)))
         self._work = torch.empty(_lib.query("cc_stage3_lp_work_size", self.n_alpha, self.P, self.d, self.groups),
-                                 dtype=torch.float32, device=sc.device)
+                                 dtype=torch.float32, device=self._slabs[0].device)
         self._ptrs_a = _lib.ptr_array([t.data_ptr() for t in self._slabs[:len(a_list)]])
         self._ptrs_b = _lib.ptr_array([t.data_ptr() for t in self._slabs[len(a_list):]])
         h = ctypes.c_void_p()
@@ -91,8 +101,7 @@ class MixedSolver:
         self.factor = factor if factor is not None else NormalizationFactor(n_operator)
         self.n = h_operator.n
         h_operator._ensure_fused()
-        n_operator._ensure_fused()
-        self._hplan, self._nplan = h_operator.plan_handle(), n_operator.plan_handle()
+        self._hplan = h_operator.plan_handle()     # N is applied through the formed N_env (factor.apply_n)
         self._lz_work, self._lz_max = None, 0
         self._jd_work = empty((_lib.query("cc_jd_work_size", self.n),))
 
@@ -128,7 +137,7 @@ class MixedSolver:
         for outer in range(max_outer):
             # FP64 H and N on the newest basis vector, then the projected pencil
             _lib.call("cc_stage3_plan_apply", stream(), self._hplan, ptr(U[k]), ptr(HU[k]))
-            _lib.call("cc_stage3_plan_apply", stream(), self._nplan, ptr(U[k]), ptr(NU[k]))
+            f.apply_n(U[k], NU[k])
             fp64_passes += 1
             k += 1
             hs, ns = empty((k, k)), empty((k, k))
@@ -193,7 +202,7 @@ class MixedSolver:
         U[0].copy_(x)
         for k in range(1, outer + 1):
             _lib.call("cc_stage3_plan_apply", stream(), self._hplan, ptr(U[k - 1]), ptr(HU[k - 1]))
-            _lib.call("cc_stage3_plan_apply", stream(), self._nplan, ptr(U[k - 1]), ptr(NU[k - 1]))
+            f.apply_n(U[k - 1], NU[k - 1])
             hs, ns = empty((k, k)), empty((k, k))
             for j in range(k):
                 _lib.call("cc_basis_dots_f64", stream(), k, n, ptr(U), ptr(HU[j]), ptr(hs[:, j]), k)

@@ -56,6 +56,11 @@ class NormalizationFactor:
         self.Wt = empty(L.shape)
         _lib.call("cc_transpose_f64", stream(), self.n_env, ptr(self.W), ptr(self.Wt))
 
+    def apply_n(self, x, y):
+        """y = (N_env (x) I_d) x for flat device vectors: the normalization multiplier through the formed
+        matrix (one skinny product) instead of the stage-3 kernel."""
+        _lib.call("cc_skinny_f64", stream(), self.n_env, 0, ptr(self.N_env), self.d, ptr(x), ptr(y))
+
 
 class GroundStateSolver:
     """Lowest eigenpair of (H, N) for one environment, FP64 on the GPU."""
@@ -92,7 +97,7 @@ class GroundStateSolver:
         """(x^T H x) / (x^T N x) in FP64 on the device."""
         hx, nx = torch.empty_like(x), torch.empty_like(x)
         self.H.apply_flat(x, hx)
-        self.N.apply_flat(x, nx)
+        self.factor.apply_n(x, nx)
         out = empty((2,))
         _lib.call("cc_dot_f64", stream(), self.n, ptr(x), ptr(hx), ptr(out[0:1]))
         _lib.call("cc_dot_f64", stream(), self.n, ptr(x), ptr(nx), ptr(out[1:2]))

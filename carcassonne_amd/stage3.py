@@ -12,6 +12,7 @@ HIP library: the fused MFMA kernel for the instantiated slab shapes, the generic
 GEMM path for any other shape.
 """
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -36,6 +37,77 @@ def cost_of_matrix(s2_0_shape, s2_1_shape, d):
     return int(m * x * y + m * d * d)
 
 
+class _Slab:
+    """One slab channel of a stage-3 operator: the stage-2 tensor t read as slab[alpha][p'][p] with
+    which 0: alpha = (t0, t1) (stage2_0.join((0,1),4,5,2,3)), which 1: alpha = (t1, t0)
+    (stage2_1.join((1,0),4,5,2,3)); p' = (t4, t5), p = (t2, t3).  The FP64 packing for the fused kernel
+    and the low-precision conversion read t in place through strides; the joined (alpha, P, P) array
+    is formed only on request (generic path, normalization matrix).  One _Slab per (tensor, which),
+    cached on the tensor, so operators built from the same stage-2 tensors share packed channels."""
+
+    def __init__(self, t, which):
+        # the channel is cached on t: hold t only weakly (no reference cycle, so the stage-2 tensor and
+        # its packed copies are freed as soon as the last operator using them goes)
+        self._tref, self.which = weakref.ref(t), which
+        self._base, self._st = t.strided_source()
+        self.shape = tuple(int(x) for x in t.shape)
+        if len(self.shape) != 6:
+            raise ValueError("a stage-2 tensor has six legs, not {}".format(len(self.shape)))
+        a0, a1 = (0, 1) if which == 0 else (1, 0)
+        self.na0, self.na1 = self.shape[a0], self.shape[a1]
+        self.n_alpha = self.na0 * self.na1
+        self.P = self.shape[2] * self.shape[3]
+        self.strided = self.shape[2] == self.shape[4] and self.shape[3] == self.shape[5]
+        self._joined, self._packed, self.lowprec = None, None, {}
+
+    @staticmethod
+    def of(t, which):
+        cache = t.__dict__.setdefault("_stage3_slabs", {})
+        if which not in cache:
+            cache[which] = _Slab(t, which)
+        return cache[which]
+
+    @property
+    def t(self):
+        t = self._tref()
+        if t is None:
+            raise RuntimeError("the stage-2 tensor of this slab channel no longer exists")
+        return t
+
+    def source(self):
+        """(base tensor, (na0, na1, d0, d1), six strides) of the slab map."""
+        st = self._st
+        a0, a1 = (0, 1) if self.which == 0 else (1, 0)
+        return self._base, (self.na0, self.na1, self.shape[4], self.shape[5]), [st[a0], st[a1], st[4], st[5], st[2], st[3]]
+
+    @property
+    def joined(self):
+        if self._joined is None:
+            if self.strided:
+                base, (na0, na1, d0, d1), st = self.source()
+                out = empty((self.n_alpha, d0 * d1, self.P))
+                _lib.call("cc_permute", stream(), _lib.CC_F64, ptr(base), ptr(out), 6,
+                          _lib.i64_array([na0, na1, d0, d1, d0, d1]), _lib.i64_array(st))
+                self._joined = HipData(out)
+            else:
+                t = self.t
+                self._joined = t.join((0, 1), 4, 5, 2, 3) if self.which == 0 else t.join((1, 0), 4, 5, 2, 3)
+        return self._joined
+
+    @property
+    def _t(self):
+        return self.joined._t
+
+    def packed(self):
+        """The channel in the fused kernel's MFMA-operand order (cc_stage3_pack_strided_f64)."""
+        if self._packed is None:
+            buf = empty((_lib.query("cc_stage3_packed_size", self.n_alpha, self.P),))
+            base, (na0, na1, d0, d1), st = self.source()
+            _lib.call("cc_stage3_pack_strided_f64", stream(), na0, na1, d0, d1, _lib.i64_array(st), ptr(base), ptr(buf))
+            self._packed = buf
+        return self._packed
+
+
 class Stage3Operator:
     """terms: list of (stage2_0, stage2_1, site_operator or None) with HipData tensors."""
 
@@ -50,7 +122,6 @@ class Stage3Operator:
         self.n = int(np.prod(self.state_shape))
         self.P, self.Q = s0.shape[2] * s0.shape[3], s1.shape[2] * s1.shape[3]
         self.n_alpha = s0.shape[0] * s0.shape[1]
-        self._joined = {}
         self._A, self._B, self._O = [], [], []
         for (a, b, op) in self.terms:
             if a.shape[0] * a.shape[1] != self.n_alpha or b.shape[0] * b.shape[1] != self.n_alpha:
@@ -58,13 +129,13 @@ class Stage3Operator:
             self._A.append(self._join(a, 0))
             self._B.append(self._join(b, 1))
             self._O.append(op)
-        self.cost_of_multiply = sum(cost_of_term(self.n_alpha, a.shape[1] * a.shape[2], self.P, b.shape[1] * b.shape[2],
+        self.cost_of_multiply = sum(cost_of_term(self.n_alpha, a.shape[4] * a.shape[5], self.P, b.shape[4] * b.shape[5],
                                                  self.Q, self.d, op is not None)
                                     for a, b, op in zip(self._A, self._B, self._O))
         self.cost_of_formMatrix = sum(cost_of_matrix(a.shape, b.shape, self.d) for (a, b, _) in self.terms)
         self._work = None
         self._fused = None
-        self.use_fused = (self.P == self.Q and
+        self.use_fused = (self.P == self.Q and all(x.strided for x in self._A + self._B) and
                           bool(_lib.load().cc_stage3_fused_supported(self.P, self.d)) and
                           self._plan_fits())
 
@@ -94,12 +165,7 @@ class Stage3Operator:
                 o = o_ids[id(O)]
             rows.append((a_ids[id(A)], k, b_ids[id(B)], o))
         rows.sort()
-        size = _lib.query("cc_stage3_packed_size", self.n_alpha, self.P)
-        packs = []
-        for ch in a_list + b_list:
-            buf = empty((size,))
-            _lib.call("cc_stage3_pack_f64", stream(), self.n_alpha, self.P, ptr(ch._t), ptr(buf))
-            packs.append(buf)
+        packs = [ch.packed() for ch in a_list + b_list]
         self._fused = dict(
             A=packs[:len(a_list)], B=packs[len(a_list):],
             A_ptrs=_lib.ptr_array([t.data_ptr() for t in packs[:len(a_list)]]),
@@ -110,10 +176,7 @@ class Stage3Operator:
             work=empty((max(1, _lib.query("cc_stage3_fused_work_size", self.n_alpha, self.P, self.d)),)))
 
     def _join(self, t, which):
-        key = (id(t), which)
-        if key not in self._joined:
-            self._joined[key] = t.join((0, 1), 4, 5, 2, 3) if which == 0 else t.join((1, 0), 4, 5, 2, 3)
-        return self._joined[key]
+        return _Slab.of(t, which)
 
     def _ensure_fused(self):
         if self._fused is None:

@@ -132,6 +132,10 @@ class _Contractor:
             result = result.contractWith(slots[t], [], [])
         if self.groups is None:
             return result.extractScalar()
+        if all(len(g) == 1 for g in self.groups) and hasattr(result, "permuted_lazily"):
+            # a pure leg permutation of the last product: left to the consumer (stage-3 packing reads it
+            # in place through strides; anything else materialises it on first use)
+            return result.permuted_lazily([g[0] for g in self.groups])
         return result.join(*self.groups)
 
 

@@ -123,6 +123,13 @@ int cc_stage3_fused_supported(int64_t P, int64_t d);
 int cc_stage3_packed_size(int64_t n_alpha, int64_t P, int64_t* out);
 int cc_stage3_fused_work_size(int64_t n_alpha, int64_t P, int64_t d, int64_t* out);
 int cc_stage3_pack_f64(void* stream, int64_t n_alpha, int64_t P, const double* slabs, double* packed);
+/* The same packing read in place from a stage-2 tensor: slab[alpha][p'][p] with alpha = a0 * na1 + a1,
+ * p' = r0 * d1 + r1, p = c0 * d1 + c1 (P = d0 * d1) at element offset a0 s[0] + a1 s[1] + r0 s[2] +
+ * r1 s[3] + c0 s[4] + c1 s[5] of src (strides = s, six entries).  This replaces the join of the stage-2
+ * tensor into (alpha, P, P) slabs (stage2_0.join((0,1),4,5,2,3), stage2_1.join((1,0),4,5,2,3);
+ * tensors/_2d/dense.py:115-173) and the packing in one pass (P <= 64). */
+int cc_stage3_pack_strided_f64(void* stream, int64_t na0, int64_t na1, int64_t d0, int64_t d1, const int64_t* strides,
+                               const double* src, double* packed);
 int cc_stage3_matvec_fused_f64(void* stream, int n_a, const double* const* A_packed, int n_b,
                                const double* const* B_packed, int n_terms, const int* term_a,
                                const int* term_b, const int* term_o, int n_ops, const double* host_ops,
@@ -193,6 +200,12 @@ int cc_lanczos_f64(void* stream, const void* plan, int64_t n_env, int64_t d, con
 int cc_stage3_lp_slab_size(int64_t n_alpha, int64_t P, int64_t* out);
 int cc_stage3_lp_convert(void* stream, int dtype, int64_t n_alpha, int64_t P, const double* slab, double* scale_out,
                          void* out);
+/* cc_stage3_lp_convert reading the channel in place through the slab map of cc_stage3_pack_strided_f64
+ * (P <= 64).  src_elems: the element count of src when the map covers every element of src exactly once
+ * (the maximum is then taken over src contiguously), else 0. */
+int cc_stage3_lp_convert_strided(void* stream, int dtype, int64_t na0, int64_t na1, int64_t d0, int64_t d1,
+                                 const int64_t* strides, int64_t src_elems, const double* src, double* scale_out,
+                                 void* out);
 int cc_stage3_lp_work_size(int64_t n_alpha, int64_t P, int64_t d, int groups, int64_t* out);
 int cc_stage3_lp_plan_create(void** out, int dtype, int n_a, const void* const* A, int n_b, const void* const* B,
                              int n_terms, const int* term_a, const int* term_b, const float* term_coef,

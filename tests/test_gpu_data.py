@@ -148,3 +148,48 @@ def test_fused_matvec_matches_generic_and_oracle(gpu, D, chi, model):
         want = apply_ref(st).ravel()
         assert rel_err(y1.cpu().numpy(), want) < 1e-12
         assert rel_err(y2.cpu().numpy(), want) < 1e-12
+
+
+def test_strided_slab_packing_matches_join(gpu):
+    """cc_stage3_pack_strided_f64 / cc_stage3_lp_convert_strided read a stage-2 tensor (contiguous, or a
+    lazily permuted view of a contraction result) in place: same packed operand order as joining it to
+    (alpha, P, P) slabs (stage2_0.join((0,1),4,5,2,3), stage2_1.join((1,0),4,5,2,3)) and packing, bit for
+    bit, and the same rounded FP16 slabs and scale as the contiguous conversion."""
+    import torch
+    from carcassonne_amd import _lib
+    from carcassonne_amd.data import HipData, PermutedView
+    from carcassonne_amd.device import ptr, stream, empty
+    from carcassonne_amd.stage3 import _Slab
+    rng = _rng(21)
+    chi, D = 5, 3
+    P = D * D
+    NT, KS = (P + 15) // 16, (P + 3) // 4
+    raw = rng.standard_normal((chi, D, D, chi, D, D))            # legs (P1, P2, P3, Q0, Q2, Q3)
+    perm = [3, 0, 1, 4, 2, 5]                                    # -> (Q0, P1, P2, Q2, P3, Q3)
+    t_np = raw.transpose(perm)
+    for t in (HipData.fromArray(np.ascontiguousarray(t_np)), PermutedView(torch.from_numpy(raw).cuda(), perm)):
+        for which in (0, 1):
+            joined = (t_np.transpose(0, 1, 4, 5, 2, 3) if which == 0 else t_np.transpose(1, 0, 4, 5, 2, 3)).reshape(chi * chi, P, P)
+            want = np.zeros((chi * chi, KS, NT, 64))
+            for u in range(KS):
+                for n in range(NT):
+                    for lane in range(64):
+                        row, col = 16 * n + lane % 16, 4 * u + lane // 16
+                        if row < P and col < P:
+                            want[:, u, n, lane] = joined[:, row, col]
+            sl = _Slab(t, which)
+            np.testing.assert_array_equal(sl.packed().cpu().numpy().reshape(want.shape), want)
+            np.testing.assert_array_equal(sl.joined.toArray().reshape(joined.shape), joined)
+            # low-precision conversion: strided read equals the contiguous one
+            n_el = _lib.query("cc_stage3_lp_slab_size", chi * chi, P)
+            got, ref = torch.empty(n_el, dtype=torch.float16, device="cuda"), torch.empty(n_el, dtype=torch.float16, device="cuda")
+            sc1, sc2 = empty((513,)), empty((513,))
+            base, (na0, na1, d0, d1), st = sl.source()
+            jd = torch.from_numpy(np.ascontiguousarray(joined)).cuda()
+            _lib.call("cc_stage3_lp_convert", stream(), _lib.CC_F16, chi * chi, P, ptr(jd), ptr(sc2), ptr(ref))
+            for src_elems in (base.numel(), 0):     # contiguous maximum, and the maximum through the map
+                got.zero_()
+                _lib.call("cc_stage3_lp_convert_strided", stream(), _lib.CC_F16, na0, na1, d0, d1, _lib.i64_array(st),
+                          src_elems, ptr(base), ptr(sc1), ptr(got))
+                assert float(sc1[0]) == float(sc2[0]) == float(np.abs(joined).max())
+                assert torch.equal(got, ref)
