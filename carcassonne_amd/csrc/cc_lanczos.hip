@@ -26,9 +26,12 @@ __global__ void __launch_bounds__(256) lz_coeff_kernel(int64_t n, const double* 
 
 // w[i] -= sum_j c[j] Q[j][i]   (or z[i] = sum_j c[j] Q[j][i] when `assign`): block = 16 outputs x
 // 16 basis slices, slice sums combined in a fixed order.
+// w -= Q^T c (assign: w = Q^T c) over the first m basis vectors; record (when given) receives c[m - 1]
 __global__ void __launch_bounds__(256) lz_combine_kernel(int m, int64_t n, const double* __restrict__ Q,
-                                                         const double* __restrict__ c, double* __restrict__ w, int assign) {
+                                                         const double* __restrict__ c, double* __restrict__ w, int assign,
+                                                         double* __restrict__ record = nullptr) {
   __shared__ double red[16][16];
+  if (record != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *record = c[m - 1];
   const int e = threadIdx.x & 15, slice = threadIdx.x >> 4;
   const int64_t i = (int64_t)blockIdx.x * 16 + e;
   double acc = 0.0;
@@ -45,13 +48,6 @@ __global__ void __launch_bounds__(256) lz_combine_kernel(int m, int64_t n, const
 }
 
 // w -= alpha q_j + beta q_{j-1}  (alpha, beta read from device memory)
-__global__ void lz_three_term_kernel(int64_t n, double* __restrict__ w, const double* __restrict__ qj,
-                                     const double* __restrict__ qjm1, const double* __restrict__ alpha,
-                                     const double* __restrict__ beta) {
-  const double a = *alpha, b = beta ? *beta : 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    w[i] -= a * qj[i] + (beta ? b * qjm1[i] : 0.0);
-}
 
 // beta = ||w||, q = w / beta (single 1024-thread block, fixed-order tree)
 __global__ void __launch_bounds__(1024) lz_normalize_kernel(int64_t n, const double* __restrict__ w, double* __restrict__ q,
@@ -73,17 +69,26 @@ __global__ void __launch_bounds__(1024) lz_normalize_kernel(int64_t n, const dou
 
 constexpr int LZ_LDS_M = 640;
 
-__device__ int sturm_count(int m, const double* a, const double* b, double x, double pivmin) {
-  int cnt = 0;
-  double q = a[0] - x;
-  if (fabs(q) < pivmin) q = -pivmin;
-  cnt += q < 0.0;
+// True when T - x I is positive definite, i.e. x lies below the lowest eigenvalue of the tridiagonal
+// (a, b): every leading principal minor is positive.  The minors follow p_{i+1} = (a_i - x) p_i -
+// b_{i-1}^2 p_{i-1} (one dependent FMA per step, no division); every fourth step both are rescaled by
+// the same power of two, which changes no sign.  b2[i] = b[i]^2.
+__device__ bool below_spectrum(int m, const double* a, const double* b2, double x) {
+  double pm = 1.0, p = a[0] - x;
+  bool pd = p > 0.0;
   for (int i = 1; i < m; ++i) {
-    q = a[i] - x - b[i - 1] * b[i - 1] / q;
-    if (fabs(q) < pivmin) q = -pivmin;
-    cnt += q < 0.0;
+    const double pn = (a[i] - x) * p - b2[i - 1] * pm;
+    pm = p;
+    p = pn;
+    pd = pd && (p > 0.0);
+    if ((i & 3) == 0) {
+      int e;
+      (void)frexp(p, &e);
+      p = ldexp(p, -e);
+      pm = ldexp(pm, -e);
+    }
   }
-  return cnt;
+  return pd;
 }
 
 // Lowest eigenpair of the m x m tridiagonal (a, b): multisection over one wave, then inverse
@@ -93,17 +98,22 @@ __global__ void __launch_bounds__(64) lz_tridiag_kernel(int m, const double* __r
                                                         double* __restrict__ scratch, int want_vector) {
   const int lane = threadIdx.x;
   // stage the tridiagonal in LDS (every lane reads all of it in the Sturm sequences)
-  __shared__ double lds_ab[2 * LZ_LDS_M];
+  __shared__ double lds_ab[3 * LZ_LDS_M];
   const bool in_lds = m <= LZ_LDS_M;
   if (in_lds) {
     for (int i = lane; i < m; i += 64) {
       lds_ab[i] = ga[i];
       lds_ab[LZ_LDS_M + i] = gb[i];
+      lds_ab[2 * LZ_LDS_M + i] = gb[i] * gb[i];
     }
+    __syncthreads();
+  } else {
+    for (int i = lane; i < m; i += 64) scratch[i] = gb[i] * gb[i];   // b^2 past the LDS capacity
     __syncthreads();
   }
   const double* a = in_lds ? lds_ab : ga;
   const double* b = in_lds ? lds_ab + LZ_LDS_M : gb;
+  const double* b2 = in_lds ? lds_ab + 2 * LZ_LDS_M : scratch;
   double lo = INFINITY, hi = -INFINITY, tnorm = 0.0;
   for (int i = lane; i < m; i += 64) {
     double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < m - 1 ? fabs(b[i]) : 0.0);
@@ -124,8 +134,7 @@ __global__ void __launch_bounds__(64) lz_tridiag_kernel(int m, const double* __r
   if (!isnan(out[0])) hi = fmin(hi, out[0] + 4.0 * eps * tnorm + pivmin);
   for (int round = 0; round < 20; ++round) {
     double x = lo + (hi - lo) * (double)(lane + 1) / 65.0;
-    int cnt = sturm_count(m, a, b, x, pivmin);
-    unsigned long long mask = __ballot(cnt >= 1);
+    unsigned long long mask = __ballot(!below_spectrum(m, a, b2, x));
     if (mask == 0ULL) {
       lo = lo + (hi - lo) * 64.0 / 65.0;
     } else {
@@ -273,7 +282,6 @@ static int lanczos_driver(hipStream_t st, ApplyFn apply, const void* plan, int64
   double* s = c + (max_iter + 2);
   double* scratch = s + (max_iter + 2);  // 5 * (max_iter + 2)
   double* res = scratch + 5 * (max_iter + 2);
-  const unsigned gb = cc_blocks(n, 256);
   int rc;
 
   // q0 = L^T x0 / ||L^T x0||
@@ -291,14 +299,12 @@ static int lanczos_driver(hipStream_t st, ApplyFn apply, const void* plan, int64
     double* qj = Q + (int64_t)j * n;
     rc = op_whitened(st, apply, plan, wh, qj, t1, t2, w);
     if (rc) return rc;
-    rc = cc_dot_f64(st, n, qj, w, alpha + j);
-    if (rc) return rc;
-    lz_three_term_kernel<<<gb, 256, 0, st>>>(n, w, qj, j > 0 ? qj - n : nullptr, alpha + j, j > 0 ? beta + j - 1 : nullptr);
-    CC_LAUNCH_CHECK();
+    // full reorthogonalisation, two classical Gram-Schmidt passes against q_0..q_j; the first pass's
+    // coefficient on q_j is alpha_j = q_j . A q_j (the three-term recurrence is contained in it)
     for (int pass = 0; pass < 2; ++pass) {
       lz_coeff_kernel<<<j + 1, 256, 0, st>>>(n, Q, w, c);
       CC_LAUNCH_CHECK();
-      lz_combine_kernel<<<(unsigned)((n + 15) / 16), 256, 0, st>>>(j + 1, n, Q, c, w, 0);
+      lz_combine_kernel<<<(unsigned)((n + 15) / 16), 256, 0, st>>>(j + 1, n, Q, c, w, 0, pass == 0 ? alpha + j : nullptr);
       CC_LAUNCH_CHECK();
     }
     lz_normalize_kernel<<<1, 1024, 0, st>>>(n, w, Q + (int64_t)(j + 1) * n, beta + j);

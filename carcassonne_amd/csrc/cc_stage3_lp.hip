@@ -18,8 +18,9 @@
 // The FP32 result of stage A (D[i][j] in lane j + 16(i/4), register i%4) is the stage-B A operand
 // (A[i][k] in lane i + 16(k/8), element k%8) once converted, provided K runs in the order of lp_kidx
 // (the B operands are read in the same order), so W never leaves registers.  V^T fragments
-// come pre-scaled from a one-workgroup prepare kernel.  With SPLIT the vector and the stage-B
-// intermediate are carried as hi + lo pairs (two MFMAs each), so only the slabs are rounded.  Each
+// come pre-scaled from a one-workgroup prepare kernel.  Split mode BOTH carries the vector and the
+// stage-B intermediate as hi + lo pairs (two MFMAs each), so only the slabs are rounded; mode VECTOR
+// splits only the vector (stage B at one MFMA per product); mode NONE rounds every operand once.  Each
 // workgroup writes one FP32 partial; a second kernel sums them in FP64 in a fixed order.
 #include "cc_common.h"
 
@@ -29,6 +30,9 @@
 namespace {
 constexpr int LP_MAX_CH = 16, LP_MAX_TERMS = 48, LP_MAX_D = 4;
 constexpr int LP_MAX_CHUNKS = 16, LP_CHUNK_SLOTS = 32;
+// Which operands carry a hi + lo pair: none (every operand rounded once), the vector and the stage-B
+// intermediate (only the slabs rounded), or the vector only (the stage-B intermediate rounded once).
+enum { LP_SPLIT_NONE = 0, LP_SPLIT_BOTH = 1, LP_SPLIT_VECTOR = 2 };
 typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) void lds_void;
@@ -188,7 +192,7 @@ __device__ __attribute__((always_inline)) inline void lp_stage(const LPChunk* __
 // tile s*QT + 2t + 1, zero past QT).  Per term: Z[s] = W[s] B over K tiles of q, then
 // Y[s'] += sum_s C_t[s'][s] Z[s] (only the diagonal when C_t is diagonal), which keeps the per-term
 // vector work to the coefficient combine.
-template <int P, int DP, typename T, bool SPLIT>
+template <int P, int DP, typename T, int SPLIT>
 __device__ __attribute__((always_inline)) inline void lp_alpha(const LPDevPlan* __restrict__ plan, const LPChunk* __restrict__ C, int n_runs,
                                 const T* __restrict__ lds, const typename LPType<T>::vec8 (&vt)[DP * ((P + 15) / 16)][(P + 31) / 32],
                                 const typename LPType<T>::vec8 (&vl)[DP * ((P + 15) / 16)][(P + 31) / 32],
@@ -210,7 +214,7 @@ __device__ __attribute__((always_inline)) inline void lp_alpha(const LPDevPlan* 
         const V bop = lds8<T, P>(As, 16 * np + (lane & 15), t, g);
 #pragma unroll
         for (int m = 0; m < MT; ++m) w[m] = LT::mfma(vt[m][t], bop, w[m]);
-        if (SPLIT) {
+        if (SPLIT != LP_SPLIT_NONE) {
 #pragma unroll
           for (int m = 0; m < MT; ++m) w[m] = LT::mfma(vl[m][t], bop, w[m]);
         }
@@ -223,12 +227,17 @@ __device__ __attribute__((always_inline)) inline void lp_alpha(const LPDevPlan* 
           for (int e = 0; e < 8; e += 2) {
             const int m = e < 4 ? 2 * t : 2 * t + 1;
             const float x0 = m < QT ? w[s * QT + m][e & 3] : 0.0f, x1 = m < QT ? w[s * QT + m][(e & 3) + 1] : 0.0f;
-            T h0, h1, l0, l1;
-            LT::split2(x0, x1, h0, h1, l0, l1);
-            whi[s][t][e] = h0;
-            whi[s][t][e + 1] = h1;
-            wlo[s][t][e] = l0;
-            wlo[s][t][e + 1] = l1;
+            if (SPLIT == LP_SPLIT_BOTH) {
+              T h0, h1, l0, l1;
+              LT::split2(x0, x1, h0, h1, l0, l1);
+              whi[s][t][e] = h0;
+              whi[s][t][e + 1] = h1;
+              wlo[s][t][e] = l0;
+              wlo[s][t][e + 1] = l1;
+            } else {  // one rounding to nearest
+              whi[s][t][e] = LT::cvt(x0);
+              whi[s][t][e + 1] = LT::cvt(x1);
+            }
           }
     }
     const int t1 = C->run_t0[r + 1];
@@ -249,7 +258,7 @@ __device__ __attribute__((always_inline)) inline void lp_alpha(const LPDevPlan* 
         for (int nq = 0; nq < NT; ++nq)
 #pragma unroll
           for (int s = 0; s < DP; ++s) z[nq][s] = LT::mfma(whi[s][t], bop[nq], z[nq][s]);
-        if (SPLIT) {
+        if (SPLIT == LP_SPLIT_BOTH) {
 #pragma unroll
           for (int nq = 0; nq < NT; ++nq)
 #pragma unroll
@@ -274,7 +283,7 @@ __device__ __attribute__((always_inline)) inline void lp_alpha(const LPDevPlan* 
   }
 }
 
-template <int P, int DP, typename T, bool SPLIT>
+template <int P, int DP, typename T, int SPLIT>
 __global__ void __launch_bounds__(64 * ((P + 15) / 16), 2) lp_kernel(const LPDevPlan* __restrict__ plan,
                                                                      const typename LPType<T>::vec8* __restrict__ fhi,
                                                                      const typename LPType<T>::vec8* __restrict__ flo,
@@ -300,7 +309,7 @@ __global__ void __launch_bounds__(64 * ((P + 15) / 16), 2) lp_kernel(const LPDev
 #pragma unroll
     for (int t = 0; t < KT; ++t) {
       vt[m][t] = fhi[(m * KT + t) * 64 + lane];
-      if (SPLIT) vl[m][t] = flo[(m * KT + t) * 64 + lane];
+      if (SPLIT != LP_SPLIT_NONE) vl[m][t] = flo[(m * KT + t) * 64 + lane];
       else vl[m][t] = V{};
     }
   f32x4 y[NT][DP];
@@ -482,8 +491,9 @@ int launch_lp(hipStream_t st, const LPHandle* h, const double* v, double* y) {
   hipEvent_t e0 = nullptr;
   int rc = cc_prof_start(st, &e0);
   if (rc != CC_OK) return rc;
-  if (h->split) lp_kernel<P, DP, T, true><<<grid, 64 * NT, 0, st>>>(plan, fhi, flo, partial);
-  else lp_kernel<P, DP, T, false><<<grid, 64 * NT, 0, st>>>(plan, fhi, flo, partial);
+  if (h->split == LP_SPLIT_BOTH) lp_kernel<P, DP, T, LP_SPLIT_BOTH><<<grid, 64 * NT, 0, st>>>(plan, fhi, flo, partial);
+  else if (h->split == LP_SPLIT_VECTOR) lp_kernel<P, DP, T, LP_SPLIT_VECTOR><<<grid, 64 * NT, 0, st>>>(plan, fhi, flo, partial);
+  else lp_kernel<P, DP, T, LP_SPLIT_NONE><<<grid, 64 * NT, 0, st>>>(plan, fhi, flo, partial);
   CC_LAUNCH_CHECK();
   rc = cc_prof_stop(st, CC_PROF_LP, e0);
   if (rc != CC_OK) return rc;
@@ -642,6 +652,7 @@ extern "C" int cc_stage3_lp_plan_create(void** out, int dtype, int n_a, const vo
                                         int split, float* work) {
   *out = nullptr;
   if (!(dtype == CC_F16 || dtype == CC_BF16)) { cc_set_last_error("lp plan: dtype must be f16 or bf16"); return CC_ERR_ARG; }
+  if (split < LP_SPLIT_NONE || split > LP_SPLIT_VECTOR) { cc_set_last_error("lp plan: split must be 0, 1 or 2"); return CC_ERR_ARG; }
   if (!cc_stage3_fused_supported(P, d)) { cc_set_last_error("lp plan: shape not instantiated"); return CC_ERR_UNSUPPORTED; }
   if (n_a < 1 || n_a > LP_MAX_CH || n_b < 1 || n_b > LP_MAX_CH || n_terms < 1 || n_terms > LP_MAX_TERMS || groups < 1 ||
       n_alpha < 1 || d > LP_MAX_D) {
@@ -667,7 +678,7 @@ extern "C" int cc_stage3_lp_plan_create(void** out, int dtype, int n_a, const vo
   h->dtype = dtype;
   h->groups = groups;
   h->chunks = dp->n_chunks;
-  h->split = split ? 1 : 0;
+  h->split = split;
   h->work = work;
   delete dp;
   if (e != hipSuccess) {

@@ -3,7 +3,11 @@ eigensolver, FP64 for everything that fixes the answer.
 
 LowPrecisionOperator -- the stage-3 operator of one solve with every distinct slab channel
     scaled by 1/max|slab| and rounded once to FP16 or BF16 (per-alpha stride padded to 16 bytes);
-    the scales fold into each term's d x d site-operator coefficients.
+    the scales fold into each term's d x d site-operator coefficients.  By default the vector is
+    carried as a hi + lo pair and the stage-B intermediate is rounded once (split "vector"): at
+    D=6, chi=64 that is as accurate as splitting both (the slab rounding dominates: first FP64
+    energy 2.36e-7 vs 2.37e-7 off) at half the stage-B matrix-core work, while rounding the vector
+    too leaves the refinement stuck near 1e-5.
 MixedSolver -- (1) Lanczos on the whitened low-precision operator (the FP64 whitening of the
     exact N) until the Ritz value moves by less than 1e-6 relative between checks: the rounded
     slabs bound the accuracy of its eigenvector (the FP64 energy of that vector is ~1e-7 off at
@@ -24,10 +28,13 @@ from .device import empty, ptr, stream
 from .solver import NormalizationFactor
 
 DTYPES = {"f16": _lib.CC_F16, "bf16": _lib.CC_BF16}
+# which operands of the low-precision kernel carry hi + lo pairs: True / "both" -- the vector and the
+# stage-B intermediate (only the slabs are rounded); "vector" -- the vector only; False / "none" -- none
+SPLIT_MODES = {True: 1, "both": 1, "vector": 2, False: 0, "none": 0}
 
 
 class LowPrecisionOperator:
-    def __init__(self, operator, dtype="bf16", groups=None, split=True):
+    def __init__(self, operator, dtype="bf16", groups=None, split="vector"):
         if not operator.use_fused:
             raise NotImplementedError("the low-precision kernel needs P = Q in {4, 9, 16, 25, 36} and d = 2")
         self.base, self.dtype = operator, dtype
@@ -74,7 +81,7 @@ class LowPrecisionOperator:
         h = ctypes.c_void_p()
         _lib.call("cc_stage3_lp_plan_create", ctypes.byref(h), code, len(a_list), self._ptrs_a, len(b_list), self._ptrs_b,
                   len(rows), _lib.i32_array([r[0] for r in rows]), _lib.i32_array([r[2] for r in rows]),
-                  _lib.f32_array(coef), self.n_alpha, self.P, self.d, self.groups, 1 if split else 0, ptr(self._work))
+                  _lib.f32_array(coef), self.n_alpha, self.P, self.d, self.groups, SPLIT_MODES[split], ptr(self._work))
         self.plan = h
         chunks = ctypes.c_int(0)
         _lib.call("cc_stage3_lp_plan_chunks", h, ctypes.byref(chunks))
@@ -95,7 +102,7 @@ class LowPrecisionOperator:
 class MixedSolver:
     """Lowest eigenpair of (H, N) to FP64 accuracy with low-precision matvecs doing the bulk."""
 
-    def __init__(self, h_operator, n_operator, dtype="bf16", factor=None, groups=None, split=True):
+    def __init__(self, h_operator, n_operator, dtype="bf16", factor=None, groups=None, split="vector"):
         self.H, self.N = h_operator, n_operator
         self.Hlp = LowPrecisionOperator(h_operator, dtype, groups=groups, split=split)
         self.factor = factor if factor is not None else NormalizationFactor(n_operator)

@@ -193,8 +193,9 @@ int cc_lanczos_f64(void* stream, const void* plan, int64_t n_env, int64_t d, con
  * cc_stage3_lp_convert: out = slab / max|slab| rounded to dtype (slab is the dense FP64 (n_alpha, P, P)
  * channel); scale_out (513 doubles of scratch) receives max|slab| in scale_out[0].
  * cc_stage3_lp_plan_create: term_coef[t] is the d x d matrix scale_a * scale_b * O_t (identity for a
- * normalization-type term); split != 0 carries the vector and the intermediate as hi + lo pairs (only
- * the slabs are rounded); `work` holds cc_stage3_lp_work_size floats and receives the device copy of
+ * normalization-type term); split 1 carries the vector and the intermediate as hi + lo pairs (only
+ * the slabs are rounded), split 2 only the vector (the intermediate is rounded once), split 0 neither;
+ * `work` holds cc_stage3_lp_work_size floats and receives the device copy of
  * the plan.  The terms are cut into chunks whose slabs fit the kernel's LDS buffers;
  * cc_stage3_lp_plan_chunks reports how many.  Apply computes y (FP64) = H_lp x (FP64). */
 int cc_stage3_lp_slab_size(int64_t n_alpha, int64_t P, int64_t* out);

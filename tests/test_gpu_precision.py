@@ -22,7 +22,7 @@ def _operators(D, chi, model, seed):
     return H.operator, N.operator, st
 
 
-@pytest.mark.parametrize("split", [False, True])
+@pytest.mark.parametrize("split", [False, True, "vector"])
 @pytest.mark.parametrize("dtype,tol", [("bf16", 2e-2), ("f16", 3e-3)])
 @pytest.mark.parametrize("D,chi", [(2, 8), (4, 16), (5, 12), (6, 20)])
 def test_lowprec_matvec_close_to_fp64(gpu, dtype, tol, D, chi, split):
