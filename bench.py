@@ -161,7 +161,8 @@ def main():
         out["mixed"] = dict(ms=ms, energy=r["energy"], err=abs(r["energy"] - e_ref) / abs(e_ref),
                             iterations=r["iterations"], info=r["info"], kernel_ms=kernel_ms, launches=nl[1],
                             fp64_kernel_ms=kms[0] / max(1, nl[0]), fp64_launches=nl[0], bytes=lp_bytes,
-                            gbs=lp_bytes / (kernel_ms * 1e-3) / 1e9, flops=pipeline.flops_per_matvec(r["operator"]))
+                            gbs=lp_bytes / (kernel_ms * 1e-3) / 1e9, flops=pipeline.flops_per_matvec(r["operator"]),
+                            kernel="lp_kernel<36,2,{},{}-split>".format(lp.dtype, lp.split))
 
     head = out.get("mixed") or out["fp64"]
     form = "mixed-f16" if "mixed" in out else "fp64-lanczos"
@@ -191,8 +192,8 @@ def main():
             line["speedup_vs_plain_fp64"] = out["fp64"]["ms"] / m["ms"]
         line["roofline"] = {"bound": "hbm", "achieved": m["gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
                             "frac": m["gbs"] / PEAK_HBM_GBS,
-                            "traffic": measured_traffic("lp_kernel<36,2,f16,split> D=6 chi=64"),
-                            "kernel": "lp_kernel<36,2,f16,split> (stage-3 matvec, FP16 slabs)",
+                            "traffic": measured_traffic(m["kernel"] + " D=6 chi=64"),
+                            "kernel": m["kernel"] + " (stage-3 matvec, FP16 slabs)",
                             "algorithmic_bytes_per_launch": m["bytes"]}
     else:
         f = out["fp64"]

@@ -147,73 +147,78 @@ __global__ void __launch_bounds__(64) lz_tridiag_kernel(int m, const double* __r
     if (hi - lo <= 2.0 * eps * fmax(fabs(lo), fabs(hi)) + pivmin) break;
   }
   const double theta = 0.5 * (lo + hi);
-  if (lane != 0) return;
   if (!want_vector) {
-    out[0] = theta;
-    out[1] = NAN;
+    if (lane == 0) {
+      out[0] = theta;
+      out[1] = NAN;
+    }
     return;
   }
-  // LU scratch lives in LDS when it fits (the common case), else in global memory
-  __shared__ double lds_scr[6 * LZ_LDS_M];
+  // One step of inverse iteration on T - theta I from a fixed start vector: theta is the eigenvalue to
+  // a couple of ulps, so one solve already gives the eigenvector.  Lane 0 runs the LU with partial
+  // pivoting (the tridiagonal form of LAPACK gttrf) fused with the forward substitution, carrying the
+  // running row in registers and storing reciprocal pivots; the back substitution is then a short FMA
+  // chain, and the whole wave normalises.  Scratch lives in LDS when it fits, else in global memory.
+  __shared__ double lds_scr[4 * LZ_LDS_M];
   double* base = (m <= LZ_LDS_M) ? lds_scr : scratch;
-  double* d = base;
-  double* dl = base + m;
-  double* du = base + 2 * m;
-  double* du2 = base + 3 * m;
-  double* piv = base + 4 * m;
-  const double tiny = eps * fmax(tnorm, 1e-300);
-  double* sv = (m <= LZ_LDS_M) ? base + 5 * m : s;  // the eigenvector, copied out at the end
-  for (int i = 0; i < m; ++i) sv[i] = 1.0 + 0.01 * (double)((i * 7919) % 101) / 101.0;
-  for (int it = 0; it < 2; ++it) {
-    for (int i = 0; i < m; ++i) {
-      d[i] = a[i] - theta;
-      if (i < m - 1) { dl[i] = b[i]; du[i] = b[i]; }
-      du2[i] = 0.0;
-      piv[i] = 0.0;
-    }
+  double* rd = base;            // 1 / pivot of row i
+  double* du = base + m;        // first superdiagonal of U
+  double* du2 = base + 2 * m;   // second superdiagonal of U (from row swaps)
+  double* sv = (m <= LZ_LDS_M) ? base + 3 * m : s;
+  __syncthreads();              // every lane is done reading b^2 from the scratch before it is reused
+  if (lane == 0) {
+    const double tiny = eps * fmax(tnorm, 1e-300);
+    auto start = [](int i) { return 1.0 + 0.01 * (double)((i * 7919) % 101) / 101.0; };
+    double cd = a[0] - theta, cdu = m > 1 ? b[0] : 0.0, cs = start(0);
     for (int i = 0; i < m - 1; ++i) {
-      if (fabs(d[i]) >= fabs(dl[i])) {
-        if (d[i] == 0.0) d[i] = tiny;
-        double f = dl[i] / d[i];
-        dl[i] = f;
-        d[i + 1] -= f * du[i];
+      const double dn = a[i + 1] - theta, dli = b[i], dun = (i + 1 < m - 1) ? b[i + 1] : 0.0, ns = start(i + 1);
+      if (fabs(cd) >= fabs(dli)) {
+        if (cd == 0.0) cd = tiny;
+        const double f = dli / cd;
+        rd[i] = 1.0 / cd;
+        du[i] = cdu;
+        du2[i] = 0.0;
+        sv[i] = cs;
+        cd = dn - f * cdu;
+        cdu = dun;
+        cs = ns - f * cs;
       } else {
-        double f = d[i] / dl[i];
-        d[i] = dl[i];
-        dl[i] = f;
-        double t = du[i];
-        du[i] = d[i + 1];
-        d[i + 1] = t - f * d[i + 1];
-        if (i < m - 2) {
-          du2[i] = du[i + 1];
-          du[i + 1] = -f * du[i + 1];
-        }
-        piv[i] = 1.0;
+        const double f = cd / dli;
+        rd[i] = 1.0 / dli;
+        du[i] = dn;
+        du2[i] = dun;
+        sv[i] = ns;
+        cd = cdu - f * dn;
+        cdu = -f * dun;
+        cs = cs - f * ns;
       }
     }
-    if (d[m - 1] == 0.0) d[m - 1] = tiny;
-    for (int i = 0; i < m - 1; ++i) {
-      if (piv[i] == 0.0) {
-        sv[i + 1] -= dl[i] * sv[i];
-      } else {
-        double t = sv[i];
-        sv[i] = sv[i + 1];
-        sv[i + 1] = t - dl[i] * sv[i];
-      }
+    if (cd == 0.0) cd = tiny;
+    rd[m - 1] = 1.0 / cd;
+    double x1 = cs * rd[m - 1], x2 = 0.0;
+    sv[m - 1] = x1;
+    for (int i = m - 2; i >= 0; --i) {
+      const double x = (sv[i] - du[i] * x1 - du2[i] * x2) * rd[i];
+      sv[i] = x;
+      x2 = x1;
+      x1 = x;
     }
-    sv[m - 1] /= d[m - 1];
-    if (m > 1) sv[m - 2] = (sv[m - 2] - du[m - 2] * sv[m - 1]) / d[m - 2];
-    for (int i = m - 3; i >= 0; --i) sv[i] = (sv[i] - du[i] * sv[i + 1] - du2[i] * sv[i + 2]) / d[i];
-    double nrm = 0.0, big = 0.0;
-    for (int i = 0; i < m; ++i) big = fmax(big, fabs(sv[i]));
-    for (int i = 0; i < m; ++i) { sv[i] /= big; nrm += sv[i] * sv[i]; }
-    nrm = sqrt(nrm);
-    for (int i = 0; i < m; ++i) sv[i] /= nrm;
   }
-  if (sv != s)
-    for (int i = 0; i < m; ++i) s[i] = sv[i];
-  out[0] = theta;
-  out[1] = fabs(b[m - 1] * sv[m - 1]);
+  __syncthreads();
+  double big = 0.0;
+  for (int i = lane; i < m; i += 64) big = fmax(big, fabs(sv[i]));
+  for (int o = 32; o > 0; o >>= 1) big = fmax(big, __shfl_xor(big, o, 64));
+  const double ib = big > 0.0 ? 1.0 / big : 1.0;
+  double nrm = 0.0;
+  for (int i = lane; i < m; i += 64) { const double v = sv[i] * ib; nrm += v * v; }
+  for (int o = 32; o > 0; o >>= 1) nrm += __shfl_xor(nrm, o, 64);
+  const double sc = ib / sqrt(nrm);
+  for (int i = lane; i < m; i += 64) s[i] = sv[i] * sc;
+  __syncthreads();
+  if (lane == 0) {
+    out[0] = theta;
+    out[1] = fabs(b[m - 1] * s[m - 1]);
+  }
 }
 
 __global__ void set_nan_kernel(double* p, int n) {

@@ -37,7 +37,7 @@ class LowPrecisionOperator:
     def __init__(self, operator, dtype="bf16", groups=None, split="vector"):
         if not operator.use_fused:
             raise NotImplementedError("the low-precision kernel needs P = Q in {4, 9, 16, 25, 36} and d = 2")
-        self.base, self.dtype = operator, dtype
+        self.base, self.dtype, self.split = operator, dtype, {1: "both", 2: "vector", 0: "none"}[SPLIT_MODES[split]]
         self.n, self.n_alpha, self.P, self.d = operator.n, operator.n_alpha, operator.P, operator.d
         code = DTYPES[dtype]
         a_ids, b_ids, a_list, b_list, rows = {}, {}, [], [], []
