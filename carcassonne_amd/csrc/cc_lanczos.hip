@@ -76,7 +76,8 @@ constexpr int LZ_LDS_M = 640;
 __device__ bool below_spectrum(int m, const double* a, const double* b2, double x) {
   double pm = 1.0, p = a[0] - x;
   bool pd = p > 0.0;
-  for (int i = 1; i < m; ++i) {
+#pragma unroll 8
+  for (int i = 1; i < m; ++i) {  // unrolled: the loads of a and b2 run ahead of the recurrence
     const double pn = (a[i] - x) * p - b2[i - 1] * pm;
     pm = p;
     p = pn;
@@ -170,6 +171,7 @@ __global__ void __launch_bounds__(64) lz_tridiag_kernel(int m, const double* __r
     const double tiny = eps * fmax(tnorm, 1e-300);
     auto start = [](int i) { return 1.0 + 0.01 * (double)((i * 7919) % 101) / 101.0; };
     double cd = a[0] - theta, cdu = m > 1 ? b[0] : 0.0, cs = start(0);
+#pragma unroll 4
     for (int i = 0; i < m - 1; ++i) {
       const double dn = a[i + 1] - theta, dli = b[i], dun = (i + 1 < m - 1) ? b[i + 1] : 0.0, ns = start(i + 1);
       if (fabs(cd) >= fabs(dli)) {
@@ -197,6 +199,7 @@ __global__ void __launch_bounds__(64) lz_tridiag_kernel(int m, const double* __r
     rd[m - 1] = 1.0 / cd;
     double x1 = cs * rd[m - 1], x2 = 0.0;
     sv[m - 1] = x1;
+#pragma unroll 4
     for (int i = m - 2; i >= 0; --i) {
       const double x = (sv[i] - du[i] * x1 - du2[i] * x2) * rd[i];
       sv[i] = x;
