@@ -97,6 +97,8 @@ class BatchedSolve:
             solver = MixedSolver(H.operator, N.operator, dtype=self.dtype, groups=self.tuning[key]["groups"])
             self.systems.append(dict(spec=spec, H=H, N=N, solver=solver, x0=x0, key=key))
         synchronize()
+        for sysm in self.systems:
+            sysm["solver"].factor.check()
 
     def prepare(self):
         self.load()

@@ -17,8 +17,8 @@
 //   stage B  Y[p' in tile np][q'][s'] += sum_q X[p'][q] B_b[q'][q],  X = sum_s C_t[s'][s] W[s*Q16 + q]
 // The FP32 result of stage A (D[i][j] in lane j + 16(i/4), register i%4) is the stage-B A operand
 // (A[i][k] in lane i + 16(k/8), element k%8) once converted, provided K runs in the order of lp_kidx
-// (the B operands are read in the same order), so W never leaves registers.  V^T fragments
-// come pre-scaled from a one-workgroup prepare kernel.  Split mode BOTH carries the vector and the
+// (the B operands are read in the same order), so W never leaves registers.  Each workgroup builds
+// the V^T fragments itself (vector staged in LDS, scaled by 1/max|v|).  Split mode BOTH carries the vector and the
 // stage-B intermediate as hi + lo pairs (two MFMAs each), so only the slabs are rounded; mode VECTOR
 // splits only the vector (stage B at one MFMA per product); mode NONE rounds every operand once.  Each
 // workgroup writes one FP32 partial; a second kernel sums them in FP64 in a fixed order.
@@ -65,6 +65,8 @@ struct LPDevPlan {
   LPChunk chunk[LP_MAX_CHUNKS];
   int term_slot[LP_MAX_TERMS];     // LDS slot of each (chunk-ordered) term's B slab
   int term_diag[LP_MAX_TERMS];     // 1 when the term's coefficient matrix is diagonal
+  int term_a[LP_MAX_TERMS];        // the term's A and B channel indices (input order = chunk order)
+  int term_b[LP_MAX_TERMS];
   float coef[LP_MAX_TERMS][LP_MAX_D * LP_MAX_D];
 };
 
@@ -131,43 +133,6 @@ template <typename T, int P>
 __device__ inline typename LPType<T>::vec8 lds8(const T* s, int row, int t, int g) {
   const typename LPType<T>::vec4 a = lds4<T, P>(s, row, 32 * t + 4 * g), b = lds4<T, P>(s, row, 32 * t + 16 + 4 * g);
   return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7);
-}
-
-// Stage-A operand fragments of the vector, shared by every wave: V^T[idx][p] with idx = s*16*QT + q,
-// scaled by 1/max|v| and split into hi + lo parts of the low-precision type, in MFMA A-operand order:
-// frag[(m*KT + t)*64 + lane][e] = V^T[16m + lane%16][lp_kidx(t, lane/16, e)] (zero outside).
-// One workgroup; max|v| (fixed-order tree) goes to scale[0] for the final rescale.
-template <int P, int DP, typename T>
-__global__ void __launch_bounds__(1024) lp_prepare_kernel(const double* __restrict__ v, typename LPType<T>::vec8* __restrict__ fhi,
-                                                          typename LPType<T>::vec8* __restrict__ flo, double* __restrict__ scale) {
-  typedef LPType<T> LT;
-  constexpr int QT = (P + 15) / 16, KT = (P + 31) / 32, MT = DP * QT, NY = P * P * DP;
-  __shared__ float red[1024];
-  float mx = 0.f;
-  for (int i = threadIdx.x; i < NY; i += 1024) mx = fmaxf(mx, (float)fabs(v[i]));
-  red[threadIdx.x] = mx;
-  __syncthreads();
-  for (int st = 512; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + st]);
-    __syncthreads();
-  }
-  const float m0 = red[0] > 0.f ? red[0] : 1.f;
-  const double vinv = 1.0 / (double)m0;
-  if (threadIdx.x == 0) scale[0] = (double)m0;
-  for (int f = threadIdx.x; f < MT * KT * 64; f += 1024) {
-    const int lane = f & 63, mt = f >> 6, m = mt / KT, t = mt % KT;
-    const int s = m / QT, q = 16 * (m % QT) + (lane & 15);
-    typename LT::vec8 hi, lo;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int p = lp_kidx(t, lane >> 4, e);
-      const float x = (q < P && p < P) ? (float)(v[(p * P + q) * DP + s] * vinv) : 0.0f;
-      hi[e] = LT::cvt(x);
-      lo[e] = LT::cvt(x - LT::round(x));
-    }
-    fhi[f] = hi;
-    flo[f] = lo;
-  }
 }
 
 // Direct global->LDS staging of one alpha's chunk slabs: wave w takes slots w, w + NT, ...; each
@@ -285,33 +250,61 @@ __device__ __attribute__((always_inline)) inline void lp_alpha(const LPDevPlan* 
 
 template <int P, int DP, typename T, int SPLIT>
 __global__ void __launch_bounds__(64 * ((P + 15) / 16), 2) lp_kernel(const LPDevPlan* __restrict__ plan,
-                                                                     const typename LPType<T>::vec8* __restrict__ fhi,
-                                                                     const typename LPType<T>::vec8* __restrict__ flo,
+                                                                     const double* __restrict__ v,
+                                                                     double* __restrict__ scale,
                                                                      float* __restrict__ partial) {
-  typedef typename LPType<T>::vec8 V;
+  typedef LPType<T> LT;
+  typedef typename LT::vec8 V;
   constexpr int NT = (P + 15) / 16, KT = (P + 31) / 32, MT = DP * NT, NY = P * P * DP;
   constexpr int CPS = lp_stride(P) * 2 / 16, BUF = lp_slots(P) * CPS + lp_pad(P);
+  static_assert(NY * 8 <= BUF * 16, "the vector must fit one slab buffer");
   __shared__ uint4 buf0[BUF];
   __shared__ uint4 buf1[BUF];
+  __shared__ float wmax[NT];
   const int lane = threadIdx.x & 63;
-  for (int i = threadIdx.x; i < BUF; i += 64 * NT) {
-    buf0[i] = uint4{0u, 0u, 0u, 0u};
-    buf1[i] = uint4{0u, 0u, 0u, 0u};
-  }
-  __syncthreads();
   const int np = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const LPChunk* C = &plan->chunk[blockIdx.y];
-  const int n_slots = C->n_slots, n_runs = C->n_runs;
-
+  // The vector: staged in buf1 as doubles, scaled by 1/max|v| (the maximum of the FP32 magnitudes,
+  // exact in any order, so every workgroup finds the same scale) and split into this lane's stage-A
+  // operand fragments V^T[16m + lane%16][lp_kidx(t, lane/16, e)] (hi + lo in the split modes).  buf1 is
+  // zeroed afterwards: the slab buffers must hold zeros past the slabs.
+  double* vs = reinterpret_cast<double*>(buf1);
+  for (int i = threadIdx.x; i < NY; i += 64 * NT) vs[i] = v[i];
+  for (int i = threadIdx.x; i < BUF; i += 64 * NT) buf0[i] = uint4{0u, 0u, 0u, 0u};
+  __syncthreads();
+  float mx = 0.f;
+  for (int i = threadIdx.x; i < NY; i += 64 * NT) mx = fmaxf(mx, (float)fabs(vs[i]));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+  if (lane == 0) wmax[np] = mx;
+  __syncthreads();
+  float m0 = 0.f;
+#pragma unroll
+  for (int w = 0; w < NT; ++w) m0 = fmaxf(m0, wmax[w]);
+  if (!(m0 > 0.f)) m0 = 1.f;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) scale[0] = (double)m0;
+  const double vinv = 1.0 / (double)m0;
   V vt[MT][KT], vl[MT][KT];
 #pragma unroll
   for (int m = 0; m < MT; ++m)
 #pragma unroll
     for (int t = 0; t < KT; ++t) {
-      vt[m][t] = fhi[(m * KT + t) * 64 + lane];
-      if (SPLIT != LP_SPLIT_NONE) vl[m][t] = flo[(m * KT + t) * 64 + lane];
-      else vl[m][t] = V{};
+      const int s = m / NT, q = 16 * (m % NT) + (lane & 15);
+      V hi, lo;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int p = lp_kidx(t, lane >> 4, e);
+        const float x = (q < P && p < P) ? (float)(vs[(p * P + q) * DP + s] * vinv) : 0.0f;
+        hi[e] = LT::cvt(x);
+        lo[e] = LT::cvt(x - LT::round(x));
+      }
+      vt[m][t] = hi;
+      vl[m][t] = (SPLIT != LP_SPLIT_NONE) ? lo : V{};
     }
+  __syncthreads();
+  for (int i = threadIdx.x; i < BUF; i += 64 * NT) buf1[i] = uint4{0u, 0u, 0u, 0u};
+  __syncthreads();
+  const LPChunk* C = &plan->chunk[blockIdx.y];
+  const int n_slots = C->n_slots, n_runs = C->n_runs;
   f32x4 y[NT][DP];
 #pragma unroll
   for (int j = 0; j < NT; ++j)
@@ -450,20 +443,18 @@ __global__ void __launch_bounds__(256) lp_convert_kernel(int64_t n_alpha, int64_
   }
 }
 
-// Workspace layout (in floats): device plan | partials [rows][n] | vector fragments hi, lo [MT*KT*64]
-// 16-byte vectors each | scale (one double).  Pieces start on 16-byte boundaries.
+// Workspace layout (in floats): device plan | partials [rows][n] | scale (one double, the vector's
+// max|v|, written by the matvec kernel).  Pieces start on 16-byte boundaries.
 struct LPLayout {
-  int64_t plan, partial, fhi, flo, scale, total;
+  int64_t plan, partial, scale, total;
 };
 inline int64_t lp_align4(int64_t x) { return (x + 3) & ~3LL; }
 inline LPLayout lp_layout(int64_t P, int64_t d, int64_t rows) {
-  const int64_t n = P * P * d, frags = d * ((P + 15) / 16) * ((P + 31) / 32) * 64;
+  const int64_t n = P * P * d;
   LPLayout L;
   L.plan = 0;
   L.partial = lp_align4((int64_t)(sizeof(LPDevPlan) + 3) / 4);
-  L.fhi = lp_align4(L.partial + rows * n);
-  L.flo = L.fhi + 4 * frags;
-  L.scale = L.flo + 4 * frags;
+  L.scale = lp_align4(L.partial + rows * n);
   L.total = L.scale + 4;
   return L;
 }
@@ -471,7 +462,9 @@ inline LPLayout lp_layout(int64_t P, int64_t d, int64_t rows) {
 struct LPHandle {
   int64_t P, d;
   int dtype, groups, chunks, split;
-  float* work;  // lp_layout(P, d, groups * LP_MAX_CHUNKS); the device plan sits at its start
+  float* work;         // lp_layout(P, d, groups * LP_MAX_CHUNKS); the device plan sits at its start
+  LPDevPlan* host;     // the host plan, kept until destroy (source of the stream-ordered upload)
+  hipEvent_t uploaded; // recorded after the upload; destroy waits for it before freeing `host`
 };
 
 template <int P, int DP, typename T>
@@ -482,18 +475,14 @@ int launch_lp(hipStream_t st, const LPHandle* h, const double* v, double* y) {
   const LPLayout L = lp_layout(P, DP, (int64_t)h->groups * LP_MAX_CHUNKS);
   const LPDevPlan* plan = reinterpret_cast<const LPDevPlan*>(h->work + L.plan);
   float* partial = h->work + L.partial;
-  V* fhi = reinterpret_cast<V*>(h->work + L.fhi);
-  V* flo = reinterpret_cast<V*>(h->work + L.flo);
   double* scale = reinterpret_cast<double*>(h->work + L.scale);
-  lp_prepare_kernel<P, DP, T><<<1, 1024, 0, st>>>(v, fhi, flo, scale);
-  CC_LAUNCH_CHECK();
   const dim3 grid((unsigned)h->groups, (unsigned)h->chunks);
   hipEvent_t e0 = nullptr;
   int rc = cc_prof_start(st, &e0);
   if (rc != CC_OK) return rc;
-  if (h->split == LP_SPLIT_BOTH) lp_kernel<P, DP, T, LP_SPLIT_BOTH><<<grid, 64 * NT, 0, st>>>(plan, fhi, flo, partial);
-  else if (h->split == LP_SPLIT_VECTOR) lp_kernel<P, DP, T, LP_SPLIT_VECTOR><<<grid, 64 * NT, 0, st>>>(plan, fhi, flo, partial);
-  else lp_kernel<P, DP, T, LP_SPLIT_NONE><<<grid, 64 * NT, 0, st>>>(plan, fhi, flo, partial);
+  if (h->split == LP_SPLIT_BOTH) lp_kernel<P, DP, T, LP_SPLIT_BOTH><<<grid, 64 * NT, 0, st>>>(plan, v, scale, partial);
+  else if (h->split == LP_SPLIT_VECTOR) lp_kernel<P, DP, T, LP_SPLIT_VECTOR><<<grid, 64 * NT, 0, st>>>(plan, v, scale, partial);
+  else lp_kernel<P, DP, T, LP_SPLIT_NONE><<<grid, 64 * NT, 0, st>>>(plan, v, scale, partial);
   CC_LAUNCH_CHECK();
   rc = cc_prof_stop(st, CC_PROF_LP, e0);
   if (rc != CC_OK) return rc;
@@ -517,6 +506,14 @@ int dispatch_lp(hipStream_t st, const LPHandle* h, const double* v, double* y) {
 }
 
 // Host: cut the (A-sorted) term table into chunks whose distinct slabs fit one LDS buffer.
+// coef[t] *= scale[a(t)] * scale[n_a + b(t)] on the device (channel scales computed by the conversion)
+__global__ void lp_scale_coef_kernel(LPDevPlan* plan, int n_terms, int dd, int n_a, const double* __restrict__ scale) {
+  for (int i = threadIdx.x; i < n_terms * dd; i += blockDim.x) {
+    const int t = i / dd, e = i % dd;
+    plan->coef[t][e] = (float)(scale[plan->term_a[t]] * scale[n_a + plan->term_b[t]] * (double)plan->coef[t][e]);
+  }
+}
+
 int lp_build_plan(LPDevPlan* dp, int64_t P, int64_t d, const void* const* A, const void* const* B, int n_terms,
                   const int* ta, const int* tb, const float* coef, int64_t n_alpha) {
   const int cap = lp_slots((int)P);
@@ -571,6 +568,8 @@ int lp_build_plan(LPDevPlan* dp, int64_t P, int64_t d, const void* const* A, con
       if (slot_of(b_ptr) < 0) slots.push_back(b_ptr);
       if ((int)slots.size() > cap) return CC_ERR_UNSUPPORTED;  // cap < 2
       dp->term_slot[out_t] = slot_of(b_ptr);
+      dp->term_a[out_t] = ta[k];
+      dp->term_b[out_t] = tb[k];
       dp->term_diag[out_t] = 1;
       for (int e = 0; e < d * d; ++e) {
         dp->coef[out_t][e] = coef[k * d * d + e];
@@ -649,7 +648,7 @@ extern "C" int cc_stage3_lp_work_size(int64_t n_alpha, int64_t P, int64_t d, int
 extern "C" int cc_stage3_lp_plan_create(void** out, int dtype, int n_a, const void* const* A, int n_b,
                                         const void* const* B, int n_terms, const int* term_a, const int* term_b,
                                         const float* term_coef, int64_t n_alpha, int64_t P, int64_t d, int groups,
-                                        int split, float* work) {
+                                        int split, float* work, void* stream, const double* channel_scales) {
   *out = nullptr;
   if (!(dtype == CC_F16 || dtype == CC_BF16)) { cc_set_last_error("lp plan: dtype must be f16 or bf16"); return CC_ERR_ARG; }
   if (split < LP_SPLIT_NONE || split > LP_SPLIT_VECTOR) { cc_set_last_error("lp plan: split must be 0, 1 or 2"); return CC_ERR_ARG; }
@@ -671,7 +670,29 @@ extern "C" int cc_stage3_lp_plan_create(void** out, int dtype, int n_a, const vo
     cc_set_last_error("lp plan: the term table needs more chunks than the kernel supports");
     return rc;
   }
-  const hipError_t e = hipMemcpy(work, dp, sizeof(LPDevPlan), hipMemcpyHostToDevice);
+  // stream-ordered upload (no host wait); the handle owns the host copy until destroy
+  hipStream_t st = cc_stream(stream);
+  hipError_t e = hipMemcpyAsync(work, dp, sizeof(LPDevPlan), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && channel_scales != nullptr) {
+    lp_scale_coef_kernel<<<1, 256, 0, st>>>(reinterpret_cast<LPDevPlan*>(work), n_terms, (int)(d * d), n_a, channel_scales);
+    e = hipGetLastError();
+  }
+  if (e != hipSuccess) {
+    hipStreamSynchronize(st);   // the copy may still read dp
+    delete dp;
+    cc_set_last_error(hipGetErrorString(e));
+    return CC_ERR_HIP;
+  }
+  hipEvent_t ev = nullptr;
+  e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventRecord(ev, st);
+  if (e != hipSuccess) {
+    hipStreamSynchronize(st);
+    if (ev) hipEventDestroy(ev);
+    delete dp;
+    cc_set_last_error(hipGetErrorString(e));
+    return CC_ERR_HIP;
+  }
   LPHandle* h = new LPHandle();
   h->P = P;
   h->d = d;
@@ -680,12 +701,8 @@ extern "C" int cc_stage3_lp_plan_create(void** out, int dtype, int n_a, const vo
   h->chunks = dp->n_chunks;
   h->split = split;
   h->work = work;
-  delete dp;
-  if (e != hipSuccess) {
-    delete h;
-    cc_set_last_error(hipGetErrorString(e));
-    return CC_ERR_HIP;
-  }
+  h->host = dp;
+  h->uploaded = ev;
   *out = h;
   return CC_OK;
 }
@@ -702,6 +719,13 @@ extern "C" int cc_stage3_lp_plan_apply(void* stream, const void* plan, const dou
 }
 
 extern "C" int cc_stage3_lp_plan_destroy(void* plan) {
-  delete static_cast<LPHandle*>(plan);
+  LPHandle* h = static_cast<LPHandle*>(plan);
+  if (!h) return CC_OK;
+  if (h->uploaded) {
+    hipEventSynchronize(h->uploaded);   // the upload has read the host plan
+    hipEventDestroy(h->uploaded);
+  }
+  delete h->host;
+  delete h;
   return CC_OK;
 }

@@ -31,6 +31,9 @@ def _prod(shape):
     return p
 
 
+_HOST_COPY_MAX = 64   # fromArray keeps a host copy of matrices up to this many elements
+
+
 def _strides(shape):
     st, acc = [], 1
     for s in reversed(shape):
@@ -69,7 +72,13 @@ class HipData(Data):
 
     @classmethod
     def fromArray(cls, a):
-        return cls(cls._upload(np.asarray(a)))
+        a = np.asarray(a)
+        out = cls(cls._upload(a))
+        if a.ndim == 2 and a.size <= _HOST_COPY_MAX and not np.iscomplexobj(a):
+            # small operands (site operators) keep their host values: reading them back (plan building)
+            # would otherwise wait for the device; in-place updates drop the copy
+            out._host = np.array(a, dtype=np.float64)
+        return out
 
     @classmethod
     def newCollected(cls, datas):
@@ -147,6 +156,9 @@ class HipData(Data):
         return self._t.numel()
 
     def toArray(self):
+        host = self.__dict__.get("_host")
+        if host is not None:
+            return host.copy()
         return self._t.cpu().numpy()
 
     def toNDArrayData(self):
@@ -175,12 +187,14 @@ class HipData(Data):
         return self._binary(other, 3)
 
     def __iadd__(self, other):
+        self.__dict__.pop("_host", None)
         if other.shape != self.shape:
             raise ValueError("shape mismatch in +=: {} {}".format(self.shape, other.shape))
         _lib.call("cc_axpby_f64", stream(), self._t.numel(), 1.0, ptr(other._t), 1.0, ptr(self._t))
         return self
 
     def __imul__(self, other):
+        self.__dict__.pop("_host", None)
         r = self * other
         self._t = r._t
         return self
@@ -215,6 +229,7 @@ class HipData(Data):
         return HipData(self._t[index].contiguous())
 
     def __setitem__(self, index, value):
+        self.__dict__.pop("_host", None)
         self._t[index] = value._t
 
     def __repr__(self):

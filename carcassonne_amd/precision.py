@@ -64,15 +64,15 @@ class LowPrecisionOperator:
                 _lib.call("cc_stage3_lp_convert_strided", stream(), code, na0, na1, d0, d1, _lib.i64_array(st),
                           base.numel() if exact else 0, ptr(base), ptr(sc[i]), ptr(buf))
                 bufs.append(buf)
-            for ch, buf, scale in zip(fresh, bufs, sc[:, 0].tolist()):
-                ch.lowprec[dtype] = (buf, scale)
+            for i, (ch, buf) in enumerate(zip(fresh, bufs)):
+                ch.lowprec[dtype] = (buf, sc[i, 0:1])          # the scale stays on the device
         self._slabs = [ch.lowprec[dtype][0] for ch in a_list + b_list]
-        scales = [ch.lowprec[dtype][1] for ch in a_list + b_list]
-        sa, sb = scales[:len(a_list)], scales[len(a_list):]
+        # channel scales gathered on the device; the plan applies them to the unscaled coefficients
+        self._scales = torch.cat([ch.lowprec[dtype][1] for ch in a_list + b_list])
         coef = []
         for (a, _, b, O) in rows:
             m = np.identity(self.d) if O is None else O.toArray()
-            coef.extend((sa[a] * sb[b] * m).ravel().tolist())
+            coef.extend(m.ravel().tolist())
         self.groups = int(groups if groups is not None else max(1, min(self.n_alpha // 8, 2048)))
         self._work = torch.empty(_lib.query("cc_stage3_lp_work_size", self.n_alpha, self.P, self.d, self.groups),
                                  dtype=torch.float32, device=self._slabs[0].device)
@@ -81,7 +81,8 @@ class LowPrecisionOperator:
         h = ctypes.c_void_p()
         _lib.call("cc_stage3_lp_plan_create", ctypes.byref(h), code, len(a_list), self._ptrs_a, len(b_list), self._ptrs_b,
                   len(rows), _lib.i32_array([r[0] for r in rows]), _lib.i32_array([r[2] for r in rows]),
-                  _lib.f32_array(coef), self.n_alpha, self.P, self.d, self.groups, SPLIT_MODES[split], ptr(self._work))
+                  _lib.f32_array(coef), self.n_alpha, self.P, self.d, self.groups, SPLIT_MODES[split], ptr(self._work),
+                  stream(), ptr(self._scales))
         self.plan = h
         chunks = ctypes.c_int(0)
         _lib.call("cc_stage3_lp_plan_chunks", h, ctypes.byref(chunks))
@@ -180,9 +181,8 @@ class MixedSolver:
                 tmp = empty((n,))
                 _lib.call("cc_basis_combine_f64", stream(), k, n, ptr(U), ptr(c), ptr(tmp))
                 _lib.call("cc_axpby_f64", stream(), n, -1.0, ptr(tmp), 1.0, ptr(t))
-            nrm = empty((1,))
-            _lib.call("cc_dot_f64", stream(), n, ptr(t), ptr(t), ptr(nrm))
-            _lib.call("cc_axpby_f64", stream(), n, 1.0 / np.sqrt(float(nrm.item())), ptr(t), 0.0, ptr(U[k]))
+            _lib.call("cc_normalize_f64", stream(), n, ptr(t), ptr(U[k]))
+        f.check()
         return xr, rho, dict(lowprec_iterations=lp_iters, outer_steps=k, fp64_passes=fp64_passes,
                              lowprec_ritz=float(res[0]), lowprec_residual=float(res[1]), history=history)
 

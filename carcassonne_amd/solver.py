@@ -48,13 +48,21 @@ class NormalizationFactor:
         work = empty((_lib.query("cc_linalg_work_size", self.n_env),))
         self.W = empty(L.shape)
         _lib.call("cc_cholesky_inverse_f64", stream(), self.n_env, ptr(L), ptr(self.W), ptr(info), ptr(work))
-        if int(info.item()) != 0:
-            raise ValueError("the normalization matrix is not positive definite (pivot {})".format(int(info.item())))
+        self._info, self._checked = info, False   # read by check() where the solver synchronizes anyway
         self.L = L
         self.Lt = empty(L.shape)
         _lib.call("cc_transpose_f64", stream(), self.n_env, ptr(L), ptr(self.Lt))
         self.Wt = empty(L.shape)
         _lib.call("cc_transpose_f64", stream(), self.n_env, ptr(self.W), ptr(self.Wt))
+
+    def check(self):
+        """Raises ValueError when the Cholesky factorization of N_env met a non-positive pivot.  Reads one
+        int from the device, so solvers call it after their own final synchronization."""
+        if not self._checked:
+            bad = int(self._info.item())
+            if bad != 0:
+                raise ValueError("the normalization matrix is not positive definite (pivot {})".format(bad))
+            self._checked = True
 
     def apply_n(self, x, y):
         """y = (N_env (x) I_d) x for flat device vectors: the normalization multiplier through the formed
@@ -91,6 +99,7 @@ class GroundStateSolver:
         _lib.call("cc_lanczos_f64", stream(), self._plan, f.n_env, f.d, ptr(f.L), ptr(f.Lt), ptr(f.W), ptr(f.Wt),
                   ptr(x0), float(tol_residual), float(tol_energy), max_iter, int(check_every), ptr(self._work),
                   ptr(x), res)
+        f.check()
         return x, float(res[0]), float(res[1]), int(res[2])
 
     def rayleigh_quotient(self, x):

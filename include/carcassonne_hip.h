@@ -195,7 +195,9 @@ int cc_lanczos_f64(void* stream, const void* plan, int64_t n_env, int64_t d, con
  * cc_stage3_lp_plan_create: term_coef[t] is the d x d matrix scale_a * scale_b * O_t (identity for a
  * normalization-type term); split 1 carries the vector and the intermediate as hi + lo pairs (only
  * the slabs are rounded), split 2 only the vector (the intermediate is rounded once), split 0 neither;
- * `work` holds cc_stage3_lp_work_size floats and receives the device copy of
+ * the plan is uploaded on `stream` without a host wait; channel_scales (device, n_a + n_b doubles:
+ * A channels then B channels, e.g. the scale_out[0] of each conversion) may be null -- otherwise
+ * term_coef holds the unscaled O_t and the scales are applied on the device; `work` holds cc_stage3_lp_work_size floats and receives the device copy of
  * the plan.  The terms are cut into chunks whose slabs fit the kernel's LDS buffers;
  * cc_stage3_lp_plan_chunks reports how many.  Apply computes y (FP64) = H_lp x (FP64). */
 int cc_stage3_lp_slab_size(int64_t n_alpha, int64_t P, int64_t* out);
@@ -210,7 +212,8 @@ int cc_stage3_lp_convert_strided(void* stream, int dtype, int64_t na0, int64_t n
 int cc_stage3_lp_work_size(int64_t n_alpha, int64_t P, int64_t d, int groups, int64_t* out);
 int cc_stage3_lp_plan_create(void** out, int dtype, int n_a, const void* const* A, int n_b, const void* const* B,
                              int n_terms, const int* term_a, const int* term_b, const float* term_coef,
-                             int64_t n_alpha, int64_t P, int64_t d, int groups, int split, float* work);
+                             int64_t n_alpha, int64_t P, int64_t d, int groups, int split, float* work,
+                             void* stream, const double* channel_scales);
 int cc_stage3_lp_plan_chunks(const void* plan, int* out);
 int cc_stage3_lp_plan_apply(void* stream, const void* plan, const double* x, double* y);
 int cc_stage3_lp_plan_destroy(void* plan);

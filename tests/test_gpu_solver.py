@@ -133,3 +133,18 @@ def test_relax_over_control_flow_matches_oracle(gpu, D, chi, model, seed):
     assert pstats["restarts"] == ostats["restarts"] and pstats["mults"] == ostats["mults"]
     assert abs(pstats["value"] - ostats["value"].real) <= 1e-9 * abs(ostats["value"])
     assert abs(abs(np.vdot(vp.toArray().ravel(), vo)) - 1.0) < 1e-6
+
+
+def test_non_positive_normalization_is_reported_by_the_solve(gpu):
+    """The Cholesky pivot check of N_env is deferred to the solver's own synchronization; a negative
+    definite normalization operator still ends the solve with ValueError (not a silent energy)."""
+    from carcassonne_amd import synthetic as syn, stage3
+    from carcassonne_amd.tensors._2d import sparse as PS
+    from carcassonne_amd.solver import GroundStateSolver
+    from tests.oracle_bridge import to_product
+    c, s, st = syn.make_environment(2, 6, model="ising", seed=4)
+    H, N = PS.formExpectationAndNormalizationMultipliers(*to_product(c, s, syn.operator_center("ising")))
+    s0, s1, _ = N.operator.terms[0]
+    negated = stage3.Stage3Operator([(s0 * -1.0, s1, None)], physical_dimension=N.operator.d)
+    with pytest.raises(ValueError, match="not positive definite"):
+        GroundStateSolver(H.operator, negated).solve(torch.from_numpy(st.ravel()).cuda())
