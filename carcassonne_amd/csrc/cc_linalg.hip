@@ -47,25 +47,32 @@ __device__ __forceinline__ void tri_sweep(double (&r)[CB], double s, F src) {
   }
 }
 
-// Right-looking Cholesky steps K..CB-1 with lane i owning row i: the pivot comes from lane K (readlane),
-// column K is scaled by 1/sqrt(pivot), broadcast through LDS, and each lane updates its row.  A
-// non-positive pivot of a real row records its 1-based index in bad and is replaced by 1e-300.
+// Cholesky and inverse in one right-looking sweep, steps K..CB-1, lane i owning row i of the block in c.
+// Before step i, c[j] (j > i) holds the mirrored Schur complement S[j][i]; at step i that is
+// sqrt(d_i) * L[j][i], i.e. -d_i * X[j][i] for X = L^-1 (d_i the pivot), and afterwards the forward
+// substitution for column i of X applied to y = -d_i * x has the same form as the Cholesky update.  So
+// every lane, whether it still factors (i > K) or already inverts (i < K), does
+//   s = c[K] / sqrt(d_K),  c[K] = s,  c[j] -= s * L[j][K]  (j > K),
+// and lane K keeps c[K] = sqrt(d_K) with s = 0.  On exit c[j] = L[i][j] (j < i), c[i] = L[i][i] and
+// c[j] = -d_i X[j][i] (j > i).  The symmetric updates keep c_i[j] == c_j[i] bit for bit (the same two
+// factors meet in each FMA).  A non-positive pivot of a real row records its 1-based index in bad and is
+// replaced by 1e-300.
 template <int K>
-__device__ __forceinline__ void chol_steps(double (&r)[CB], double* colv, int i, int kb, int& bad) {
+__device__ __forceinline__ void chol_inv_steps(double (&c)[CB], double* colv, int i, int kb, int& bad) {
   if constexpr (K < CB) {
-    double d = readlane_f64(r[K], K);
+    double d = readlane_f64(c[K], K);
     if (!(d > 0.0)) {
       if (K < kb && bad == 0) bad = K + 1;
       d = 1e-300;
     }
     const double sd = sqrt(d), inv = 1.0 / sd;
-    const double l = (i > K) ? r[K] * inv : 0.0;
-    r[K] = (i == K) ? sd : ((i > K) ? l : r[K]);
-    colv[i] = l;
+    const double s = (i == K) ? 0.0 : c[K] * inv;
+    c[K] = (i == K) ? sd : s;
+    colv[i] = s;
     __syncthreads();
-    tri_sweep<K>(r, l, [&](int j) { return colv[j]; });
+    tri_sweep<K>(c, s, [&](int j) { return colv[j]; });
     __syncthreads();
-    chol_steps<K + 1>(r, colv, i, kb, bad);
+    chol_inv_steps<K + 1>(c, colv, i, kb, bad);
   }
 }
 
@@ -83,8 +90,8 @@ __device__ __forceinline__ void inv_steps(double (&x)[CB], const double (*Ls)[CB
 // One diagonal block of at most 64 x 64 on one wave (the workgroup), in registers.  FACTOR: Cholesky of
 // the block of A at (k0, k0) (lower triangle read; rows >= kb padded with the identity), written back to
 // A with the block's upper triangle zeroed; info receives the first non-positive pivot (1-based, global
-// index).  Without FACTOR the block of A already holds L.  Then X = L^-1 of the block into X at (x0, x0),
-// lane j solving L x = e_j with L broadcast from LDS.
+// index), and X = L^-1 of the block into X at (x0, x0), both from the one sweep of chol_inv_steps.
+// Without FACTOR the block of A already holds L, and lane j solves L x = e_j with L broadcast from LDS.
 template <bool FACTOR>
 __global__ void __launch_bounds__(64) tri_leaf_kernel(int64_t lda, int64_t k0, int kb, double* __restrict__ A,
                                                       double* __restrict__ X, int64_t ldx, int64_t x0,
@@ -93,25 +100,39 @@ __global__ void __launch_bounds__(64) tri_leaf_kernel(int64_t lda, int64_t k0, i
   __shared__ double colv[CB];
   __shared__ double rdiag[CB];
   const int i = threadIdx.x;
-  const double* arow = A + (k0 + (i < kb ? i : 0)) * lda + k0;  // clamped: every load below is in bounds
+  const int ic = i < kb ? i : 0;  // clamped: every load below is in bounds
   double r[CB];
+  if (FACTOR) {
+    // row i of the lower triangle, then column i of it mirrored into j > i (identity past kb)
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+      const int jc = j < kb ? j : 0;
+      const double v = A[(k0 + (j <= i ? ic : jc)) * lda + k0 + (j <= i ? jc : ic)];
+      r[j] = (i < kb && j < kb) ? v : (i == j ? 1.0 : 0.0);
+    }
+    int bad = 0;
+    chol_inv_steps<0>(r, colv, i, kb, bad);
+    if (i == 0 && bad && *info == 0) *info = (int)(k0 + bad);
+#pragma unroll
+    for (int j = 0; j < CB; ++j) Ls[i][j] = r[j];
+    __syncthreads();
+    for (int rr = 0; rr < kb; ++rr)
+      if (i < kb) A[(k0 + rr) * lda + k0 + i] = (i <= rr) ? Ls[rr][i] : 0.0;
+    // X[m][i] = -c_i[m] / d_i below the diagonal, 1 / L[i][i] on it
+    const double xd = 1.0 / Ls[i][i], rd = -xd * xd;
+    for (int m = 0; m < kb; ++m)
+      if (i < kb) X[(x0 + m) * ldx + x0 + i] = (m > i) ? Ls[i][m] * rd : (m == i ? xd : 0.0);
+    return;
+  }
+  const double* arow = A + (k0 + ic) * lda + k0;
 #pragma unroll
   for (int j = 0; j < CB; ++j) {
     const double v = arow[j < kb ? j : 0];
     r[j] = (i < kb && j <= i) ? v : (i == j ? 1.0 : 0.0);
   }
-  if (FACTOR) {
-    int bad = 0;
-    chol_steps<0>(r, colv, i, kb, bad);
-    if (i == 0 && bad && *info == 0) *info = (int)(k0 + bad);
-  }
 #pragma unroll
   for (int j = 0; j < CB; ++j) Ls[i][j] = (j <= i) ? r[j] : 0.0;
   __syncthreads();
-  if (FACTOR) {
-    for (int rr = 0; rr < kb; ++rr)
-      if (i < kb) A[(k0 + rr) * lda + k0 + i] = Ls[rr][i];
-  }
   rdiag[i] = 1.0 / Ls[i][i];
   __syncthreads();
   double x[CB];

@@ -257,22 +257,40 @@ __global__ void __launch_bounds__(64 * ((P + 15) / 16), 2) lp_kernel(const LPDev
   typedef typename LT::vec8 V;
   constexpr int NT = (P + 15) / 16, KT = (P + 31) / 32, MT = DP * NT, NY = P * P * DP;
   constexpr int CPS = lp_stride(P) * 2 / 16, BUF = lp_slots(P) * CPS + lp_pad(P);
-  static_assert(NY * 8 <= BUF * 16, "the vector must fit one slab buffer");
+  constexpr int NTH = 64 * NT, NV = (NY + NTH - 1) / NTH;
+  // stage-A operand fragments in LDS, vec8 [hi | lo][m][t][lane]
+  constexpr int FT = MT * KT * 64, NF = SPLIT != LP_SPLIT_NONE ? 2 : 1;
   __shared__ uint4 buf0[BUF];
   __shared__ uint4 buf1[BUF];
+  __shared__ uint4 frag[NF * FT];
   __shared__ float wmax[NT];
   const int lane = threadIdx.x & 63;
   const int np = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  // The vector: staged in buf1 as doubles, scaled by 1/max|v| (the maximum of the FP32 magnitudes,
-  // exact in any order, so every workgroup finds the same scale) and split into this lane's stage-A
-  // operand fragments V^T[16m + lane%16][lp_kidx(t, lane/16, e)] (hi + lo in the split modes).  buf1 is
-  // zeroed afterwards: the slab buffers must hold zeros past the slabs.
-  double* vs = reinterpret_cast<double*>(buf1);
-  for (int i = threadIdx.x; i < NY; i += 64 * NT) vs[i] = v[i];
-  for (int i = threadIdx.x; i < BUF; i += 64 * NT) buf0[i] = uint4{0u, 0u, 0u, 0u};
-  __syncthreads();
+  const LPChunk* C = &plan->chunk[blockIdx.y];
+  const int n_slots = C->n_slots, n_runs = C->n_runs;
+  const int64_t a0 = plan->n_alpha * blockIdx.x / gridDim.x, a1 = plan->n_alpha * (blockIdx.x + 1) / gridDim.x;
+  // The vector (all loads in flight at once; clamped index, masked value), then the first alpha's
+  // slabs, so that both latencies overlap.
+  double xv[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const int i = threadIdx.x + NTH * k;
+    xv[k] = v[i < NY ? i : NY - 1];
+  }
+  if (a0 < a1) lp_stage<P, NT>(C, n_slots, a0, buf0, np, lane);
+  // Past the staged slots the buffers hold zeros (operand reads run past the last slab), and so do the
+  // fragment elements past the vector (q or p >= P).
+  for (int i = n_slots * CPS + threadIdx.x; i < BUF; i += NTH) {
+    buf0[i] = uint4{0u, 0u, 0u, 0u};
+    buf1[i] = uint4{0u, 0u, 0u, 0u};
+  }
+  for (int i = threadIdx.x; i < NF * FT; i += NTH) frag[i] = uint4{0u, 0u, 0u, 0u};
+  // Scale 1/max|v|: the maximum of the FP32 magnitudes, exact in any order, so every workgroup finds
+  // the same scale.
   float mx = 0.f;
-  for (int i = threadIdx.x; i < NY; i += 64 * NT) mx = fmaxf(mx, (float)fabs(vs[i]));
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+    if (threadIdx.x + NTH * k < NY) mx = fmaxf(mx, (float)fabs(xv[k]));
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
   if (lane == 0) wmax[np] = mx;
@@ -283,36 +301,37 @@ __global__ void __launch_bounds__(64 * ((P + 15) / 16), 2) lp_kernel(const LPDev
   if (!(m0 > 0.f)) m0 = 1.f;
   if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) scale[0] = (double)m0;
   const double vinv = 1.0 / (double)m0;
+  // Each element once per workgroup: v[(p P + q) DP + s] is element e of lane (q % 16) + 16 g of fragment
+  // (m = s NT + q / 16, t = p / 32), where p % 32 = lp_kidx(0, g, e) (hi + lo in the split modes).
+  T* fh = reinterpret_cast<T*>(frag);
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const int i = threadIdx.x + NTH * k;
+    if (i < NY) {
+      const int sv = i % DP, pq = i / DP, q = pq % P, p = pq / P, kk = p & 31;
+      const int m = sv * NT + (q >> 4), g = (kk & 15) >> 2, e = (kk & 3) + ((kk >> 4) << 2);
+      const int idx = (((m * KT + (p >> 5)) * 64) + (q & 15) + 16 * g) * 8 + e;
+      const float x = (float)(xv[k] * vinv);
+      fh[idx] = LT::cvt(x);
+      if (NF == 2) fh[FT * 8 + idx] = LT::cvt(x - LT::round(x));
+    }
+  }
+  __syncthreads();
+  const V* fv = reinterpret_cast<const V*>(frag);
   V vt[MT][KT], vl[MT][KT];
 #pragma unroll
   for (int m = 0; m < MT; ++m)
 #pragma unroll
     for (int t = 0; t < KT; ++t) {
-      const int s = m / NT, q = 16 * (m % NT) + (lane & 15);
-      V hi, lo;
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int p = lp_kidx(t, lane >> 4, e);
-        const float x = (q < P && p < P) ? (float)(vs[(p * P + q) * DP + s] * vinv) : 0.0f;
-        hi[e] = LT::cvt(x);
-        lo[e] = LT::cvt(x - LT::round(x));
-      }
-      vt[m][t] = hi;
-      vl[m][t] = (SPLIT != LP_SPLIT_NONE) ? lo : V{};
+      vt[m][t] = fv[(m * KT + t) * 64 + lane];
+      vl[m][t] = (NF == 2) ? fv[FT + (m * KT + t) * 64 + lane] : V{};
     }
-  __syncthreads();
-  for (int i = threadIdx.x; i < BUF; i += 64 * NT) buf1[i] = uint4{0u, 0u, 0u, 0u};
-  __syncthreads();
-  const LPChunk* C = &plan->chunk[blockIdx.y];
-  const int n_slots = C->n_slots, n_runs = C->n_runs;
   f32x4 y[NT][DP];
 #pragma unroll
   for (int j = 0; j < NT; ++j)
 #pragma unroll
     for (int s = 0; s < DP; ++s) y[j][s] = (f32x4){0.f, 0.f, 0.f, 0.f};
 
-  const int64_t a0 = plan->n_alpha * blockIdx.x / gridDim.x, a1 = plan->n_alpha * (blockIdx.x + 1) / gridDim.x;
-  if (a0 < a1) lp_stage<P, NT>(C, n_slots, a0, buf0, np, lane);
   for (int64_t al = a0; al < a1; al += 2) {
     // buffer 0 holds alpha al; everyone has finished reading buffer 1 (alpha al-1)
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
