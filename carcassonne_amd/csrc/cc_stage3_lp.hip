@@ -358,36 +358,61 @@ __global__ void __launch_bounds__(64 * ((P + 15) / 16), 2) lp_kernel(const LPDev
 
 }
 
-// y[i] = max|v| * sum over partial rows in row order (FP64).  One workgroup owns 32 columns; its 8 row
-// groups (32 lanes each) sum rows g, g + 8, ... and the 8 group sums are added in group order.
-constexpr int LP_RCOLS = 32, LP_RGROUPS = 8;
-__global__ void __launch_bounds__(LP_RCOLS * LP_RGROUPS) lp_reduce_kernel(int64_t rows, int64_t n, const float* __restrict__ partial,
-                                                                         const double* __restrict__ scale, double* __restrict__ y) {
-  __shared__ double part[LP_RGROUPS][LP_RCOLS];
-  const int col = threadIdx.x % LP_RCOLS, grp = threadIdx.x / LP_RCOLS;
-  const int64_t i = (int64_t)blockIdx.x * LP_RCOLS + col;
-  double acc = 0.0;
-  if (i < n) {
-    int64_t r = grp;
-    for (; r + 3 * LP_RGROUPS < rows; r += 4 * LP_RGROUPS) {
-      const float a = partial[r * n + i], b = partial[(r + LP_RGROUPS) * n + i];
-      const float c = partial[(r + 2 * LP_RGROUPS) * n + i], d = partial[(r + 3 * LP_RGROUPS) * n + i];
-      acc += (double)a;
-      acc += (double)b;
-      acc += (double)c;
-      acc += (double)d;
+// y[i] = max|v| * sum over partial rows in row order (FP64).  A workgroup owns LP_RQ * VW columns; its
+// LP_RG row groups take consecutive row ranges, each thread loading LP_RU rows of VW columns at once,
+// and the group sums are added in group order.
+constexpr int LP_RQ = 8, LP_RG = 32, LP_RU = 16;
+template <int VW> struct LPVec;
+template <> struct LPVec<4> { typedef float4 type; static __device__ float at(const float4& v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; } };
+template <> struct LPVec<2> { typedef float2 type; static __device__ float at(const float2& v, int k) { return k == 0 ? v.x : v.y; } };
+template <> struct LPVec<1> { typedef float type; static __device__ float at(const float& v, int) { return v; } };
+template <int VW>
+__global__ void __launch_bounds__(LP_RQ * LP_RG) lp_reduce_kernel(int64_t rows, int64_t n, const float* __restrict__ partial,
+                                                                 const double* __restrict__ scale, double* __restrict__ y) {
+  typedef typename LPVec<VW>::type FV;
+  __shared__ double part[LP_RG][LP_RQ * VW];
+  const int q = threadIdx.x % LP_RQ, g = threadIdx.x / LP_RQ;
+  const int64_t c0 = ((int64_t)blockIdx.x * LP_RQ + q) * VW;
+  const int64_t rpt = (rows + LP_RG - 1) / LP_RG, r0 = g * rpt, r1 = rows < r0 + rpt ? rows : r0 + rpt;
+  double acc[VW];
+#pragma unroll
+  for (int k = 0; k < VW; ++k) acc[k] = 0.0;
+  if (c0 < n) {
+    for (int64_t rb = r0; rb < r1; rb += LP_RU) {
+      FV v[LP_RU];
+#pragma unroll
+      for (int u = 0; u < LP_RU; ++u) {  // clamped row: every load in bounds, the value masked below
+        const int64_t r = rb + u < r1 ? rb + u : r0;
+        v[u] = *reinterpret_cast<const FV*>(partial + r * n + c0);
+      }
+#pragma unroll
+      for (int u = 0; u < LP_RU; ++u)
+        if (rb + u < r1) {
+#pragma unroll
+          for (int k = 0; k < VW; ++k) acc[k] += (double)LPVec<VW>::at(v[u], k);
+        }
     }
-    for (; r < rows; r += LP_RGROUPS) acc += (double)partial[r * n + i];
   }
-  part[grp][col] = acc;
+#pragma unroll
+  for (int k = 0; k < VW; ++k) part[g][q * VW + k] = acc[k];
   __syncthreads();
-  if (grp == 0 && i < n) {
-    double t = 0.0;
-    for (int k = 0; k < LP_RGROUPS; ++k) t += part[k][col];
-    y[i] = t * scale[0];
+  if ((int)threadIdx.x < LP_RQ * VW) {
+    const int64_t i = (int64_t)blockIdx.x * LP_RQ * VW + threadIdx.x;
+    if (i < n) {
+      double t = 0.0;
+      for (int h = 0; h < LP_RG; ++h) t += part[h][threadIdx.x];
+      y[i] = t * scale[0];
+    }
   }
 }
 
+// the sum of the workgroup partials with the widest loads the row length allows (rows start 16-byte
+// aligned: the partial area is)
+inline void lp_reduce(hipStream_t st, int64_t rows, int64_t n, const float* partial, const double* scale, double* y) {
+  if (n % 4 == 0) lp_reduce_kernel<4><<<(unsigned)((n + LP_RQ * 4 - 1) / (LP_RQ * 4)), LP_RQ * LP_RG, 0, st>>>(rows, n, partial, scale, y);
+  else if (n % 2 == 0) lp_reduce_kernel<2><<<(unsigned)((n + LP_RQ * 2 - 1) / (LP_RQ * 2)), LP_RQ * LP_RG, 0, st>>>(rows, n, partial, scale, y);
+  else lp_reduce_kernel<1><<<(unsigned)((n + LP_RQ - 1) / LP_RQ), LP_RQ * LP_RG, 0, st>>>(rows, n, partial, scale, y);
+}
 
 // low-precision slab = slab / max|slab|: pass 1 per-block maxima, pass 2 (one block) their maximum
 // max |x| over n contiguous elements: block partials (four independent loads in flight per thread),
@@ -505,8 +530,7 @@ int launch_lp(hipStream_t st, const LPHandle* h, const double* v, double* y) {
   CC_LAUNCH_CHECK();
   rc = cc_prof_stop(st, CC_PROF_LP, e0);
   if (rc != CC_OK) return rc;
-  lp_reduce_kernel<<<(unsigned)((n + LP_RCOLS - 1) / LP_RCOLS), LP_RCOLS * LP_RGROUPS, 0, st>>>(
-      (int64_t)h->groups * h->chunks, n, partial, scale, y);
+  lp_reduce(st, (int64_t)h->groups * h->chunks, n, partial, scale, y);
   CC_LAUNCH_CHECK();
   return CC_OK;
 }
