@@ -47,33 +47,90 @@ __device__ __forceinline__ void tri_sweep(double (&r)[CB], double s, F src) {
   }
 }
 
-// Cholesky and inverse in one right-looking sweep, steps K..CB-1, lane i owning row i of the block in c.
-// Before step i, c[j] (j > i) holds the mirrored Schur complement S[j][i]; at step i that is
-// sqrt(d_i) * L[j][i], i.e. -d_i * X[j][i] for X = L^-1 (d_i the pivot), and afterwards the forward
-// substitution for column i of X applied to y = -d_i * x has the same form as the Cholesky update.  So
-// every lane, whether it still factors (i > K) or already inverts (i < K), does
-//   s = c[K] / sqrt(d_K),  c[K] = s,  c[j] -= s * L[j][K]  (j > K),
-// and lane K keeps c[K] = sqrt(d_K) with s = 0.  On exit c[j] = L[i][j] (j < i), c[i] = L[i][i] and
+// Cholesky and inverse of a 64 x 64 block in one right-looking sweep, row i of the block held by lane i
+// as c[j].  Before step i, c[j] (j > i) holds the mirrored Schur complement S[j][i]; at step i that is
+// sqrt(d_i) L[j][i], i.e. -d_i X[j][i] for X = L^-1 (d_i the pivot), and afterwards the forward
+// substitution for column i of X applied to y = -d_i x has the same form as the Cholesky update.  So
+// every row, whether it still factors (i > K) or already inverts (i < K), does
+//   s_i = c[K] / sqrt(d_K),  c[K] = s_i,  c[j] -= s_i s_j  (j > K),
+// and row K keeps c[K] = sqrt(d_K) with s_K = 0.  On exit c[j] = L[i][j] (j < i), c[i] = L[i][i] and
 // c[j] = -d_i X[j][i] (j > i).  The symmetric updates keep c_i[j] == c_j[i] bit for bit (the same two
 // factors meet in each FMA).  A non-positive pivot of a real row records its 1-based index in bad and is
 // replaced by 1e-300.
+// Four waves: wave w holds columns 16w .. 16w + 15 of every row.  At step K the wave owning column K
+// forms the pivot and the multipliers, publishes them in LDS (two buffers by step parity, so one barrier
+// per step), and every wave with columns past K updates them.
+constexpr int CW = 16, NW4 = CB / CW;
 template <int K>
-__device__ __forceinline__ void chol_inv_steps(double (&c)[CB], double* colv, int i, int kb, int& bad) {
+__device__ __forceinline__ void chol_inv_steps4(double (&c)[CW], double (*sbuf)[CB], int w, int i, int kb, int& bad) {
   if constexpr (K < CB) {
-    double d = readlane_f64(c[K], K);
-    if (!(d > 0.0)) {
-      if (K < kb && bad == 0) bad = K + 1;
-      d = 1e-300;
+    constexpr int o = K / CW, k = K % CW;
+    double* sb = sbuf[K & 1];
+    if (w == o) {
+      double d = readlane_f64(c[k], K);
+      if (!(d > 0.0)) {
+        if (K < kb && bad == 0) bad = K + 1;
+        d = 1e-300;
+      }
+      const double sd = sqrt(d), inv = 1.0 / sd;
+      const double sv = (i == K) ? 0.0 : c[k] * inv;
+      c[k] = (i == K) ? sd : sv;
+      sb[i] = sv;
     }
-    const double sd = sqrt(d), inv = 1.0 / sd;
-    const double s = (i == K) ? 0.0 : c[K] * inv;
-    c[K] = (i == K) ? sd : s;
-    colv[i] = s;
     __syncthreads();
-    tri_sweep<K>(c, s, [&](int j) { return colv[j]; });
-    __syncthreads();
-    chol_inv_steps<K + 1>(c, colv, i, kb, bad);
+    if (w >= o) {
+      const double si = sb[i];
+      if (w == o) {
+#pragma unroll
+        for (int jj = k + 1; jj < CW; ++jj) c[jj] -= si * sb[CW * o + jj];
+      } else {
+        const double* sw = sb + CW * w;
+#pragma unroll
+        for (int jj = 0; jj < CW; ++jj) c[jj] -= si * sw[jj];
+      }
+    }
+    chol_inv_steps4<K + 1>(c, sbuf, w, i, kb, bad);
   }
+}
+
+// Cholesky and inverse of one diagonal block of at most 64 x 64 (see chol_inv_steps4): A's block at
+// (k0, k0) (lower triangle read; rows >= kb padded with the identity) is replaced by L with its upper
+// triangle zeroed, X's block at (x0, x0) receives L^-1, info the first non-positive pivot (1-based,
+// global index).
+__global__ void __launch_bounds__(64 * NW4) chol_leaf_kernel(int64_t lda, int64_t k0, int kb, double* __restrict__ A,
+                                                           double* __restrict__ X, int64_t ldx, int64_t x0,
+                                                           int* __restrict__ info) {
+  __shared__ double Ls[CB][CB + 1];
+  __shared__ double sbuf[2][CB];
+  const int i = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int ic = i < kb ? i : 0;  // clamped: every load below is in bounds
+  double c[CW];
+  // row i of the lower triangle, then column i of it mirrored into j > i (identity past kb)
+#pragma unroll
+  for (int jj = 0; jj < CW; ++jj) {
+    const int j = CW * w + jj, jc = j < kb ? j : 0;
+    const double v = A[(k0 + (j <= i ? ic : jc)) * lda + k0 + (j <= i ? jc : ic)];
+    c[jj] = (i < kb && j < kb) ? v : (i == j ? 1.0 : 0.0);
+  }
+  int bad = 0;  // first failing pivot among this wave's columns
+  chol_inv_steps4<0>(c, sbuf, w, i, kb, bad);
+  __shared__ int badw[NW4];
+  if (i == 0) badw[w] = bad;
+#pragma unroll
+  for (int jj = 0; jj < CW; ++jj) Ls[i][CW * w + jj] = c[jj];
+  __syncthreads();
+  if (threadIdx.x == 0 && *info == 0)
+    for (int v = 0; v < NW4; ++v)
+      if (badw[v]) {
+        *info = (int)(k0 + badw[v]);
+        break;
+      }
+  for (int rr = w; rr < kb; rr += NW4)
+    if (i < kb) A[(k0 + rr) * lda + k0 + i] = (i <= rr) ? Ls[rr][i] : 0.0;
+  // X[m][i] = -c_i[m] / d_i below the diagonal, 1 / L[i][i] on it
+  const double xd = 1.0 / Ls[i][i], rd = -xd * xd;
+  for (int m = w; m < kb; m += NW4)
+    if (i < kb) X[(x0 + m) * ldx + x0 + i] = (m > i) ? Ls[i][m] * rd : (m == i ? xd : 0.0);
 }
 
 // Column sweep of L x = e_j, steps Q..CB-1: x[Q] *= 1/L[Q][Q], then x[m] -= L[m][Q] x[Q] for m > Q.
@@ -87,43 +144,16 @@ __device__ __forceinline__ void inv_steps(double (&x)[CB], const double (*Ls)[CB
   }
 }
 
-// One diagonal block of at most 64 x 64 on one wave (the workgroup), in registers.  FACTOR: Cholesky of
-// the block of A at (k0, k0) (lower triangle read; rows >= kb padded with the identity), written back to
-// A with the block's upper triangle zeroed; info receives the first non-positive pivot (1-based, global
-// index), and X = L^-1 of the block into X at (x0, x0), both from the one sweep of chol_inv_steps.
-// Without FACTOR the block of A already holds L, and lane j solves L x = e_j with L broadcast from LDS.
-template <bool FACTOR>
-__global__ void __launch_bounds__(64) tri_leaf_kernel(int64_t lda, int64_t k0, int kb, double* __restrict__ A,
-                                                      double* __restrict__ X, int64_t ldx, int64_t x0,
-                                                      int* __restrict__ info) {
+// Inverse of one lower-triangular diagonal block of at most 64 x 64 on one wave: the block of A at (k0, k0)
+// holds L (rows >= kb padded with the identity); lane j solves L x = e_j with L broadcast from LDS and
+// writes column j of L^-1 into X at (x0, x0).
+__global__ void __launch_bounds__(64) tri_leaf_kernel(int64_t lda, int64_t k0, int kb, const double* __restrict__ A,
+                                                      double* __restrict__ X, int64_t ldx, int64_t x0) {
   __shared__ double Ls[CB][CB + 1];
-  __shared__ double colv[CB];
   __shared__ double rdiag[CB];
   const int i = threadIdx.x;
   const int ic = i < kb ? i : 0;  // clamped: every load below is in bounds
   double r[CB];
-  if (FACTOR) {
-    // row i of the lower triangle, then column i of it mirrored into j > i (identity past kb)
-#pragma unroll
-    for (int j = 0; j < CB; ++j) {
-      const int jc = j < kb ? j : 0;
-      const double v = A[(k0 + (j <= i ? ic : jc)) * lda + k0 + (j <= i ? jc : ic)];
-      r[j] = (i < kb && j < kb) ? v : (i == j ? 1.0 : 0.0);
-    }
-    int bad = 0;
-    chol_inv_steps<0>(r, colv, i, kb, bad);
-    if (i == 0 && bad && *info == 0) *info = (int)(k0 + bad);
-#pragma unroll
-    for (int j = 0; j < CB; ++j) Ls[i][j] = r[j];
-    __syncthreads();
-    for (int rr = 0; rr < kb; ++rr)
-      if (i < kb) A[(k0 + rr) * lda + k0 + i] = (i <= rr) ? Ls[rr][i] : 0.0;
-    // X[m][i] = -c_i[m] / d_i below the diagonal, 1 / L[i][i] on it
-    const double xd = 1.0 / Ls[i][i], rd = -xd * xd;
-    for (int m = 0; m < kb; ++m)
-      if (i < kb) X[(x0 + m) * ldx + x0 + i] = (m > i) ? Ls[i][m] * rd : (m == i ? xd : 0.0);
-    return;
-  }
   const double* arow = A + (k0 + ic) * lda + k0;
 #pragma unroll
   for (int j = 0; j < CB; ++j) {
@@ -211,11 +241,11 @@ inline int64_t split_rows(int64_t sz) { return ((sz + CB - 1) / CB / 2) * CB; }
 //   (L11, X11) = chol_inv(A11); L21 = A21 X11^T; A22 -= L21 L21^T; (L22, X22) = chol_inv(A22);
 //   X21 = -X22 (L21 X11).
 // The GEMMs have the full K of each level (not the block width), so they fill the chip; leaves are
-// tri_leaf_kernel<true>.  tmp holds n1 x n2 <= sz^2 / 4 doubles.
+// chol_leaf_kernel.  tmp holds n1 x n2 <= sz^2 / 4 doubles.
 int chol_inv_rec(void* stream, int64_t n, int64_t off, int64_t sz, double* A, double* X, double* tmp, int* info) {
   hipStream_t st = cc_stream(stream);
   if (sz <= CB) {
-    tri_leaf_kernel<true><<<1, 64, 0, st>>>(n, off, (int)sz, A, X, n, off, info);
+    chol_leaf_kernel<<<1, 64 * NW4, 0, st>>>(n, off, (int)sz, A, X, n, off, info);
     CC_LAUNCH_CHECK();
     return CC_OK;
   }
@@ -241,11 +271,11 @@ int chol_inv_rec(void* stream, int64_t n, int64_t off, int64_t sz, double* A, do
 }
 
 // Recursive inverse of the lower-triangular range [off, off + sz) of L: X11, X22 recursively, then
-// X21 = -X22 (L21 X11); leaves are tri_leaf_kernel<false>.
+// X21 = -X22 (L21 X11); leaves are tri_leaf_kernel.
 int trtri_rec(void* stream, int64_t n, int64_t off, int64_t sz, const double* L, double* X, double* tmp) {
   hipStream_t st = cc_stream(stream);
   if (sz <= CB) {
-    tri_leaf_kernel<false><<<1, 64, 0, st>>>(n, off, (int)sz, const_cast<double*>(L), X, n, off, nullptr);
+    tri_leaf_kernel<<<1, 64, 0, st>>>(n, off, (int)sz, L, X, n, off);
     CC_LAUNCH_CHECK();
     return CC_OK;
   }
@@ -269,7 +299,7 @@ extern "C" int cc_linalg_work_size(int64_t n, int64_t* out) {
   return CC_OK;
 }
 
-// Right-looking blocked Cholesky: diagonal block factored and inverted on one wave, the panel as a GEMM
+// Right-looking blocked Cholesky: diagonal block factored and inverted by chol_leaf_kernel, the panel as a GEMM
 // against that inverse, the trailing update as a GEMM.
 extern "C" int cc_cholesky_f64(void* stream, int64_t n, double* A, int* info_dev, double* work) {
   if (n <= 0) { cc_set_last_error("cholesky: n must be positive"); return CC_ERR_ARG; }
@@ -280,7 +310,7 @@ extern "C" int cc_cholesky_f64(void* stream, int64_t n, double* A, int* info_dev
   CC_LAUNCH_CHECK();
   for (int64_t k0 = 0; k0 < n; k0 += CB) {
     int kb = (int)((n - k0) < CB ? (n - k0) : CB);
-    tri_leaf_kernel<true><<<1, 64, 0, st>>>(n, k0, kb, A, dinv, kb, 0, info_dev);
+    chol_leaf_kernel<<<1, 64 * NW4, 0, st>>>(n, k0, kb, A, dinv, kb, 0, info_dev);
     CC_LAUNCH_CHECK();
     int64_t rest = n - k0 - kb;
     if (rest > 0) {
