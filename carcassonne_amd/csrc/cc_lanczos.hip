@@ -9,19 +9,37 @@
 
 namespace {
 
-__global__ void __launch_bounds__(256) lz_coeff_kernel(int64_t n, const double* __restrict__ Q, const double* __restrict__ w,
-                                                       double* __restrict__ c) {
-  __shared__ double red[256];
-  const double* q = Q + (int64_t)blockIdx.x * n;
+// sum_i a[i] b[i] over the 256 threads of a workgroup in a fixed order: each thread loads its DOT_U
+// pairs at once (clamped index, masked product), then a tree over the threads.
+constexpr int DOT_U = 12;
+__device__ inline double block_dot256(int64_t n, const double* __restrict__ a, const double* __restrict__ b, double* red) {
   double acc = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += 256) acc += q[i] * w[i];
+  for (int64_t base = threadIdx.x; base < n; base += 256 * DOT_U) {
+    double x[DOT_U], y[DOT_U];
+#pragma unroll
+    for (int u = 0; u < DOT_U; ++u) {
+      const int64_t i = base + 256 * u < n ? base + 256 * u : base;
+      x[u] = a[i];
+      y[u] = b[i];
+    }
+#pragma unroll
+    for (int u = 0; u < DOT_U; ++u)
+      if (base + 256 * u < n) acc += x[u] * y[u];
+  }
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) c[blockIdx.x] = red[0];
+  return red[0];
+}
+
+__global__ void __launch_bounds__(256) lz_coeff_kernel(int64_t n, const double* __restrict__ Q, const double* __restrict__ w,
+                                                       double* __restrict__ c) {
+  __shared__ double red[256];
+  const double t = block_dot256(n, Q + (int64_t)blockIdx.x * n, w, red);
+  if (threadIdx.x == 0) c[blockIdx.x] = t;
 }
 
 // w[i] -= sum_j c[j] Q[j][i]   (or z[i] = sum_j c[j] Q[j][i] when `assign`): block = 16 outputs x
@@ -374,16 +392,8 @@ namespace {
 __global__ void __launch_bounds__(256) jd_dots_kernel(int64_t n, const double* __restrict__ U, const double* __restrict__ v,
                                                       double* __restrict__ out, int stride_out) {
   __shared__ double red[256];
-  const double* u = U + (int64_t)blockIdx.x * n;
-  double acc = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += 256) acc += u[i] * v[i];
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[(int64_t)blockIdx.x * stride_out] = red[0];
+  const double t = block_dot256(n, U + (int64_t)blockIdx.x * n, v, red);
+  if (threadIdx.x == 0) out[(int64_t)blockIdx.x * stride_out] = t;
 }
 
 // y = a*x + b*y with a, b read from device scalars (a = sa[0]*fa, b = sb ? sb[0]*fb : fb)
@@ -446,6 +456,60 @@ __global__ void __launch_bounds__(1024) cg_step_kernel(int64_t n, const double* 
   const double beta = rr_new / rr[0];
   for (int64_t i = threadIdx.x; i < n; i += 1024) p[i] = res[i] + beta * p[i];
   __syncthreads();
+  if (threadIdx.x == 0) rr[0] = rr_new;
+}
+
+// The same step with the vectors in registers (n <= 1024 R): one load round, three fixed-order sums,
+// one store round (ap is scratch for the next step's operator product and is not written back).
+template <int R>
+__global__ void __launch_bounds__(1024) cg_step_reg_kernel(int64_t n, const double* __restrict__ rho_dev, const double* __restrict__ u,
+                                                           const double* __restrict__ ap, double* __restrict__ p,
+                                                           double* __restrict__ y, double* __restrict__ res,
+                                                           double* __restrict__ rr) {
+  __shared__ double red[16];
+  const double rho = rho_dev[0], rr0 = rr[0];
+  double a[R], pv[R], uv[R], yv[R], rv[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) {  // clamped index: every load in bounds, the values masked in the sums
+    const int64_t i = threadIdx.x + 1024 * k, ic = i < n ? i : 0;
+    a[k] = ap[ic];
+    pv[k] = p[ic];
+    uv[k] = u[ic];
+    yv[k] = y[ic];
+    rv[k] = res[ic];
+  }
+  double du = 0.0;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    a[k] -= rho * pv[k];
+    if (threadIdx.x + 1024 * k < n) du += uv[k] * a[k];
+  }
+  const double c = cg_block_sum(du, red);
+  double pap = 0.0;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    a[k] -= c * uv[k];
+    if (threadIdx.x + 1024 * k < n) pap += pv[k] * a[k];
+  }
+  const double alpha = rr0 / cg_block_sum(pap, red);
+  double rn = 0.0;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    yv[k] += alpha * pv[k];
+    rv[k] -= alpha * a[k];
+    if (threadIdx.x + 1024 * k < n) rn += rv[k] * rv[k];
+  }
+  const double rr_new = cg_block_sum(rn, red);
+  const double beta = rr_new / rr0;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int64_t i = threadIdx.x + 1024 * k;
+    if (i < n) {
+      p[i] = rv[k] + beta * pv[k];
+      y[i] = yv[k];
+      res[i] = rv[k];
+    }
+  }
   if (threadIdx.x == 0) rr[0] = rr_new;
 }
 
@@ -594,7 +658,8 @@ extern "C" int cc_jd_correction_lp(void* stream, const void* lp_plan, int64_t n_
     // ap = P (W H W^T - rho) p   (p is kept in the projected space)
     rc = op_whitened(st, cc_internal_lp_apply, lp_plan, wh, p, t1, t2, ap);
     if (rc) return rc;
-    cg_step_kernel<<<1, 1024, 0, st>>>(n, rho_dev, u, ap, p, y, res, sc + 1);
+    if (n <= 4096) cg_step_reg_kernel<4><<<1, 1024, 0, st>>>(n, rho_dev, u, ap, p, y, res, sc + 1);
+    else cg_step_kernel<<<1, 1024, 0, st>>>(n, rho_dev, u, ap, p, y, res, sc + 1);
     CC_LAUNCH_CHECK();
   }
   if ((rc = project(y))) return rc;
