@@ -148,7 +148,7 @@ class HipData(Data):
         return cls.newFilled(shape, 0.0)
 
     # -- properties ---------------------------------------------------------------
-    shape = property(lambda self: tuple(self._t.shape))
+    shape = property(lambda self: self._t.shape)  # a torch.Size (a tuple)
     ndim = property(lambda self: self._t.dim())
     dtype = property(lambda self: np.dtype(np.float64))
 
@@ -276,10 +276,11 @@ class HipData(Data):
     # -- shape operations (cc_permute) ----------------------------------------------------
     def _permuted(self, perm):
         perm = [int(p) for p in perm]
-        if perm == list(range(self.ndim)):
+        sh = self.shape
+        if perm == list(range(len(sh))):
             return self._t
-        shape = [self.shape[p] for p in perm]
-        st = _strides(self.shape)
+        shape = [sh[p] for p in perm]
+        st = _strides(sh)
         out = empty(shape)
         if out.numel():
             _lib.call("cc_permute", stream(), _lib.CC_F64, ptr(self._t), ptr(out), len(perm),
@@ -352,19 +353,21 @@ class HipData(Data):
     # -- contraction (cc_gemm_f64) ----------------------------------------------------
     def contractWith(self, other, self_axes, other_axes):
         """tensordot(self, other, (self_axes, other_axes)) on the GPU."""
-        self_axes = [int(a) % max(self.ndim, 1) for a in self_axes]
-        other_axes = [int(a) % max(other.ndim, 1) for a in other_axes]
+        ssh, osh = self.shape, other.shape
+        sn, on = len(ssh), len(osh)
+        self_axes = [int(a) % max(sn, 1) for a in self_axes]
+        other_axes = [int(a) % max(on, 1) for a in other_axes]
         if len(self_axes) != len(other_axes):
             raise ValueError("shape-mismatch for sum")
         for a, b in zip(self_axes, other_axes):
-            if self.shape[a] != other.shape[b]:
+            if ssh[a] != osh[b]:
                 raise ValueError("shape-mismatch for sum")
-        fa = [i for i in range(self.ndim) if i not in self_axes]
-        fb = [i for i in range(other.ndim) if i not in other_axes]
-        M = _prod([self.shape[i] for i in fa])
-        K = _prod([self.shape[i] for i in self_axes])
-        N = _prod([other.shape[i] for i in fb])
-        out_shape = [self.shape[i] for i in fa] + [other.shape[i] for i in fb]
+        fa = [i for i in range(sn) if i not in self_axes]
+        fb = [i for i in range(on) if i not in other_axes]
+        M = _prod([ssh[i] for i in fa])
+        K = _prod([ssh[i] for i in self_axes])
+        N = _prod([osh[i] for i in fb])
+        out_shape = [ssh[i] for i in fa] + [osh[i] for i in fb]
         out = empty(out_shape)
         if out.numel() == 0:
             return HipData(out)
@@ -372,15 +375,15 @@ class HipData(Data):
             _lib.call("cc_fill_f64", stream(), out.numel(), 0.0, ptr(out))
             return HipData(out)
         # A as (M, K): either free-then-contracted or contracted-then-free in memory order
-        if fa + self_axes == list(range(self.ndim)):
+        if fa + self_axes == list(range(sn)):
             A, sAm, sAk = self._t, K, 1
-        elif self_axes + fa == list(range(self.ndim)):
+        elif self_axes + fa == list(range(sn)):
             A, sAm, sAk = self._t, 1, M
         else:
             A, sAm, sAk = self._permuted(fa + self_axes), K, 1
-        if other_axes + fb == list(range(other.ndim)):
+        if other_axes + fb == list(range(on)):
             B, sBk, sBn = other._t, N, 1
-        elif fb + other_axes == list(range(other.ndim)):
+        elif fb + other_axes == list(range(on)):
             B, sBk, sBn = other._t, 1, K
         else:
             B, sBk, sBn = other._permuted(other_axes + fb), N, 1

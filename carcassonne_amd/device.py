@@ -15,13 +15,20 @@ def device():
     return _DEVICE
 
 
+_DEVICE_INDEX = None
+
+
 def stream():
-    """The caller's current HIP stream as a void* for the C ABI."""
-    return ctypes.c_void_p(torch.cuda.current_stream(device()).cuda_stream)
+    """The caller's current HIP stream as a void* for the C ABI (the raw handle: this is called before
+    every kernel launch, and building a torch Stream object each time costs microseconds)."""
+    global _DEVICE_INDEX
+    if _DEVICE_INDEX is None:
+        _DEVICE_INDEX = device().index
+    return ctypes.c_void_p(torch._C._cuda_getCurrentRawStream(_DEVICE_INDEX))
 
 
 def empty(shape, dtype=torch.float64):
-    return torch.empty(tuple(int(s) for s in shape), dtype=dtype, device=device())
+    return torch.empty([int(s) for s in shape], dtype=dtype, device=_DEVICE if _DEVICE is not None else device())
 
 
 def ptr(t):
