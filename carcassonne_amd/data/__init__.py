@@ -514,6 +514,8 @@ class PermutedView(HipData):
         if not (isinstance(base, torch.Tensor) and base.is_contiguous() and base.dtype == torch.float64):
             raise TypeError("PermutedView needs a contiguous float64 device tensor")
         self._base, self._perm, self._mat = base, [int(p) for p in perm], None
+        bs = base.shape
+        self._shape = torch.Size([bs[p] for p in self._perm])
 
     @property
     def _t(self):
@@ -525,15 +527,14 @@ class PermutedView(HipData):
     @_t.setter
     def _t(self, value):
         self._mat, self._base = value, None
+        self._shape = value.shape
 
     def _lazy(self):
         return self.__dict__.get("_mat") is None and self.__dict__.get("_base") is not None
 
     @property
     def shape(self):
-        if self._lazy():
-            return tuple(int(self._base.shape[p]) for p in self._perm)
-        return tuple(self._t.shape)
+        return self._shape
 
     @property
     def ndim(self):
