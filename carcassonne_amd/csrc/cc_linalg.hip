@@ -61,28 +61,47 @@ __device__ __forceinline__ void tri_sweep(double (&r)[CB], double s, F src) {
 // forms the pivot and the multipliers, publishes them in LDS (two buffers by step parity, so one barrier
 // per step), and every wave with columns past K updates them.
 constexpr int CW = 16, NW4 = CB / CW;
+// Pivot of step K in its owner wave K / CW: the multipliers s_i into sb and c[K % CW] set.
+template <int K>
+__device__ __forceinline__ void chol_pivot4(double (&c)[CW], double* sb, int i, int kb, int& bad) {
+  constexpr int k = K % CW;
+  double d = readlane_f64(c[k], K);
+  if (!(d > 0.0)) {
+    if (K < kb && bad == 0) bad = K + 1;
+    d = 1e-300;
+  }
+  const double sd = sqrt(d), inv = 1.0 / sd;
+  const double sv = (i == K) ? 0.0 : c[k] * inv;
+  c[k] = (i == K) ? sd : sv;
+  sb[i] = sv;
+}
+
+// Step K (its multipliers already in sbuf[K & 1]): after the barrier the owner of column K + 1 applies
+// step K to that column first and forms the pivot of step K + 1 into the other buffer (look-ahead), so
+// the pivot chain overlaps the other columns' updates.
 template <int K>
 __device__ __forceinline__ void chol_inv_steps4(double (&c)[CW], double (*sbuf)[CB], int w, int i, int kb, int& bad) {
   if constexpr (K < CB) {
     constexpr int o = K / CW, k = K % CW;
-    double* sb = sbuf[K & 1];
-    if (w == o) {
-      double d = readlane_f64(c[k], K);
-      if (!(d > 0.0)) {
-        if (K < kb && bad == 0) bad = K + 1;
-        d = 1e-300;
-      }
-      const double sd = sqrt(d), inv = 1.0 / sd;
-      const double sv = (i == K) ? 0.0 : c[k] * inv;
-      c[k] = (i == K) ? sd : sv;
-      sb[i] = sv;
-    }
+    constexpr bool ahead = K + 1 < CB;
+    constexpr int o1 = ahead ? (K + 1) / CW : 0, k1 = ahead ? (K + 1) % CW : 0;
     __syncthreads();
+    const double* sb = sbuf[K & 1];
     if (w >= o) {
       const double si = sb[i];
+      if (ahead && w == o1) {
+        c[k1] -= si * sb[K + 1];
+        if constexpr (ahead) chol_pivot4<K + 1>(c, sbuf[(K + 1) & 1], i, kb, bad);
+      }
       if (w == o) {
+        // columns past K in the owner wave, less column K + 1 when the look-ahead took it
+        constexpr int j0 = (ahead && o1 == o) ? k + 2 : k + 1;
 #pragma unroll
-        for (int jj = k + 1; jj < CW; ++jj) c[jj] -= si * sb[CW * o + jj];
+        for (int jj = j0; jj < CW; ++jj) c[jj] -= si * sb[CW * o + jj];
+      } else if (ahead && o1 != o && w == o1) {
+        const double* sw = sb + CW * w;
+#pragma unroll
+        for (int jj = 1; jj < CW; ++jj) c[jj] -= si * sw[jj];
       } else {
         const double* sw = sb + CW * w;
 #pragma unroll
@@ -113,6 +132,7 @@ __global__ void __launch_bounds__(64 * NW4) chol_leaf_kernel(int64_t lda, int64_
     c[jj] = (i < kb && j < kb) ? v : (i == j ? 1.0 : 0.0);
   }
   int bad = 0;  // first failing pivot among this wave's columns
+  if (w == 0) chol_pivot4<0>(c, sbuf[0], i, kb, bad);
   chol_inv_steps4<0>(c, sbuf, w, i, kb, bad);
   __shared__ int badw[NW4];
   if (i == 0) badw[w] = bad;
