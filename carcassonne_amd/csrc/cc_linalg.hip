@@ -70,7 +70,13 @@ __device__ __forceinline__ void chol_pivot4(double (&c)[CW], double* sb, int i, 
     if (K < kb && bad == 0) bad = K + 1;
     d = 1e-300;
   }
-  const double sd = sqrt(d), inv = 1.0 / sd;
+  // 1/sqrt(d): the hardware estimate and two Newton steps y += (y/2)(1 - d y^2) (each squares the
+  // relative error; within an ulp or so of the rounded quotient), sqrt(d) = d / sqrt(d); far shorter
+  // than a correctly rounded square root followed by a division on this step's critical path
+  double inv = __builtin_amdgcn_rsq(d);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) inv = fma(0.5 * inv, fma(-d * inv, inv, 1.0), inv);
+  const double sd = d * inv;
   const double sv = (i == K) ? 0.0 : c[k] * inv;
   c[k] = (i == K) ? sd : sv;
   sb[i] = sv;
