@@ -100,6 +100,28 @@ static inline CCSlabGather cc_slab_gather(const CCSlabMap& m, int64_t d0) {
   }
   return g;
 }
+// The same with every leg of extent D (P = D * D) and NTH threads: the index arithmetic divides by
+// constants, and each thread issues all its loads before its LDS stores.
+template <int D, int NTH>
+__device__ inline void cc_slab_load_d(const CCSlabGather& g, const double* __restrict__ src, int64_t al, double* lds, int ld,
+                                      double mul) {
+  constexpr int n = D * D * D * D, NE = (n + NTH - 1) / NTH;
+  const int64_t a0 = al / g.na1, a1 = al - a0 * g.na1;
+  const double* base = src + a0 * g.sa0 + a1 * g.sa1;
+  double v[NE];
+  int at[NE];
+#pragma unroll
+  for (int u = 0; u < NE; ++u) {
+    const int e = threadIdx.x + NTH * u, ec = e < n ? e : 0;  // clamped: every load in bounds
+    const int k3 = ec % D, k2 = (ec / D) % D, k1 = (ec / (D * D)) % D, k0 = ec / (D * D * D);
+    v[u] = base[k0 * g.st[0] + k1 * g.st[1] + k2 * g.st[2] + k3 * g.st[3]];
+    at[u] = (k0 * g.rw[0] + k1 * g.rw[1] + k2 * g.rw[2] + k3 * g.rw[3]) * ld + k0 * g.cw[0] + k1 * g.cw[1] + k2 * g.cw[2] +
+            k3 * g.cw[3];
+  }
+#pragma unroll
+  for (int u = 0; u < NE; ++u)
+    if (threadIdx.x + NTH * u < n) lds[at[u]] = v[u] * mul;
+}
 // Loads slab alpha into lds[row * ld + col] (row, col < P) with all threads of the block.
 __device__ inline void cc_slab_load(const CCSlabGather& g, const double* __restrict__ src, int64_t al, double* lds, int ld,
                                     double mul) {

@@ -61,6 +61,25 @@ __global__ void __launch_bounds__(256) pack_kernel(int64_t n_alpha, int NT, int 
   }
 }
 
+// The same for slabs whose four legs all have extent D (constant index arithmetic).
+template <int D>
+__global__ void __launch_bounds__(256) pack_kernel_d(int64_t n_alpha, CCSlabGather g, const double* __restrict__ src,
+                                                     double* __restrict__ dst) {
+  constexpr int P = D * D, NT = (P + 15) / 16, KS = (P + 3) / 4, per = KS * NT * 64, ld = P + 1;
+  __shared__ double s[P * ld];
+  for (int64_t al = blockIdx.x; al < n_alpha; al += gridDim.x) {
+    cc_slab_load_d<D, 256>(g, src, al, s, ld, 1.0);
+    __syncthreads();
+    double* out = dst + al * per;
+    for (int r = threadIdx.x; r < per; r += blockDim.x) {
+      const int lane = r & 63, t = r >> 6, n = t % NT, u = t / NT;
+      const int row = 16 * n + (lane & 15), col = 4 * u + (lane >> 4);
+      out[r] = (row < P && col < P) ? s[row * ld + col] : 0.0;
+    }
+    __syncthreads();
+  }
+}
+
 template <int P, int DP>
 __global__ void __launch_bounds__(256) fused_f64_kernel(const FusedPlan plan, const double* __restrict__ v,
                                                         double* __restrict__ partial) {
@@ -298,7 +317,19 @@ extern "C" int cc_stage3_pack_strided_f64(void* stream, int64_t na0, int64_t na1
   int NT = (int)((P + 15) / 16), KS = (int)((P + 3) / 4);
   const unsigned grid = (unsigned)(n_alpha < 16384 ? n_alpha : 16384);
   const size_t lds = (size_t)P * (P + 1) * sizeof(double);
-  pack_kernel<<<grid, 256, lds, cc_stream(stream)>>>(n_alpha, NT, KS, cc_slab_gather(map, d0), src, packed);
+  const CCSlabGather g = cc_slab_gather(map, d0);
+  hipStream_t st = cc_stream(stream);
+  if (d0 == map.d1 && d0 >= 2 && d0 <= 6) {
+    switch (d0) {
+      case 2: pack_kernel_d<2><<<grid, 256, 0, st>>>(n_alpha, g, src, packed); break;
+      case 3: pack_kernel_d<3><<<grid, 256, 0, st>>>(n_alpha, g, src, packed); break;
+      case 4: pack_kernel_d<4><<<grid, 256, 0, st>>>(n_alpha, g, src, packed); break;
+      case 5: pack_kernel_d<5><<<grid, 256, 0, st>>>(n_alpha, g, src, packed); break;
+      default: pack_kernel_d<6><<<grid, 256, 0, st>>>(n_alpha, g, src, packed); break;
+    }
+  } else {
+    pack_kernel<<<grid, 256, lds, st>>>(n_alpha, NT, KS, g, src, packed);
+  }
   CC_LAUNCH_CHECK();
   return CC_OK;
 }

@@ -487,6 +487,36 @@ __global__ void __launch_bounds__(256) lp_convert_kernel(int64_t n_alpha, int64_
   }
 }
 
+// The same for slabs whose four legs all have extent D (constant index arithmetic).
+template <typename T, int D>
+__global__ void __launch_bounds__(256) lp_convert_kernel_d(int64_t n_alpha, CCSlabGather g, const double* __restrict__ x,
+                                                           const double* __restrict__ scale, T* __restrict__ y) {
+  constexpr int P = D * D, ld = P + 1, PP = P * P, SP = lp_stride(P);
+  __shared__ double s[P * ld];
+  const double inv = 1.0 / scale[0];
+  for (int64_t al = blockIdx.x; al < n_alpha; al += gridDim.x) {
+    cc_slab_load_d<D, 256>(g, x, al, s, ld, inv);
+    __syncthreads();
+    T* out = y + al * SP;
+    for (int i = threadIdx.x; i < SP; i += blockDim.x) out[i] = i < PP ? (T)s[(i / P) * ld + i % P] : (T)0.0f;
+    __syncthreads();
+  }
+}
+template <typename T>
+void launch_lp_convert(hipStream_t st, unsigned grid, size_t lds, int64_t n_alpha, int64_t SP, int64_t d0, int64_t d1,
+                       const CCSlabGather& g, const double* src, const double* scale, T* out) {
+  if (d0 == d1 && d0 >= 2 && d0 <= 6) {
+    switch (d0) {
+      case 2: lp_convert_kernel_d<T, 2><<<grid, 256, 0, st>>>(n_alpha, g, src, scale, out); return;
+      case 3: lp_convert_kernel_d<T, 3><<<grid, 256, 0, st>>>(n_alpha, g, src, scale, out); return;
+      case 4: lp_convert_kernel_d<T, 4><<<grid, 256, 0, st>>>(n_alpha, g, src, scale, out); return;
+      case 5: lp_convert_kernel_d<T, 5><<<grid, 256, 0, st>>>(n_alpha, g, src, scale, out); return;
+      default: lp_convert_kernel_d<T, 6><<<grid, 256, 0, st>>>(n_alpha, g, src, scale, out); return;
+    }
+  }
+  lp_convert_kernel<T><<<grid, 256, lds, st>>>(n_alpha, SP, g, src, scale, out);
+}
+
 // Workspace layout (in floats): device plan | partials [rows][n] | scale (one double, the vector's
 // max|v|, written by the matvec kernel).  Pieces start on 16-byte boundaries.
 struct LPLayout {
@@ -668,8 +698,8 @@ extern "C" int cc_stage3_lp_convert_strided(void* stream, int dtype, int64_t na0
   const unsigned grid = (unsigned)(n_alpha < 16384 ? n_alpha : 16384);
   const CCSlabGather g = cc_slab_gather(map, d0);
   const size_t lds = (size_t)P * (P + 1) * sizeof(double);
-  if (dtype == CC_F16) lp_convert_kernel<_Float16><<<grid, 256, lds, st>>>(n_alpha, SP, g, src, scale_out, (_Float16*)out);
-  else lp_convert_kernel<__bf16><<<grid, 256, lds, st>>>(n_alpha, SP, g, src, scale_out, (__bf16*)out);
+  if (dtype == CC_F16) launch_lp_convert<_Float16>(st, grid, lds, n_alpha, SP, d0, d1, g, src, scale_out, (_Float16*)out);
+  else launch_lp_convert<__bf16>(st, grid, lds, n_alpha, SP, d0, d1, g, src, scale_out, (__bf16*)out);
   CC_LAUNCH_CHECK();
   return CC_OK;
 }
