@@ -47,10 +47,42 @@ class Data:
         return _prod(self.shape)
 
 
+_CONTRACT_PLANS = {}
+
+
+def _contract_plan(ssh, osh, self_axes, other_axes):
+    """The shape bookkeeping of one tensordot (memoised per shapes and axes): (M, N, K, output shape,
+    A layout mode and permutation, B layout mode and permutation)."""
+    sn, on = len(ssh), len(osh)
+    self_axes = [int(a) % max(sn, 1) for a in self_axes]
+    other_axes = [int(a) % max(on, 1) for a in other_axes]
+    if len(self_axes) != len(other_axes):
+        raise ValueError("shape-mismatch for sum")
+    for a, b in zip(self_axes, other_axes):
+        if ssh[a] != osh[b]:
+            raise ValueError("shape-mismatch for sum")
+    fa = [i for i in range(sn) if i not in self_axes]
+    fb = [i for i in range(on) if i not in other_axes]
+    M = _prod([ssh[i] for i in fa])
+    K = _prod([ssh[i] for i in self_axes])
+    N = _prod([osh[i] for i in fb])
+    out_shape = [int(ssh[i]) for i in fa] + [int(osh[i]) for i in fb]
+    a_mode = 0 if fa + self_axes == list(range(sn)) else (1 if self_axes + fa == list(range(sn)) else 2)
+    b_mode = 0 if other_axes + fb == list(range(on)) else (1 if fb + other_axes == list(range(on)) else 2)
+    return M, N, K, out_shape, a_mode, fa + self_axes, b_mode, other_axes + fb
+
+
 class HipData(Data):
     """A real FP64 tensor in device memory with the NDArrayData method surface."""
 
     # -- construction -------------------------------------------------------------
+    @classmethod
+    def _wrap(cls, t):
+        """A HipData around a float64 device tensor this module just produced (no input checks)."""
+        d = HipData.__new__(HipData)
+        d._t = t
+        return d
+
     def __init__(self, t):
         if isinstance(t, np.ndarray) or np.isscalar(t):
             t = HipData._upload(np.asarray(t))
@@ -354,42 +386,36 @@ class HipData(Data):
     def contractWith(self, other, self_axes, other_axes):
         """tensordot(self, other, (self_axes, other_axes)) on the GPU."""
         ssh, osh = self.shape, other.shape
-        sn, on = len(ssh), len(osh)
-        self_axes = [int(a) % max(sn, 1) for a in self_axes]
-        other_axes = [int(a) % max(on, 1) for a in other_axes]
-        if len(self_axes) != len(other_axes):
-            raise ValueError("shape-mismatch for sum")
-        for a, b in zip(self_axes, other_axes):
-            if ssh[a] != osh[b]:
-                raise ValueError("shape-mismatch for sum")
-        fa = [i for i in range(sn) if i not in self_axes]
-        fb = [i for i in range(on) if i not in other_axes]
-        M = _prod([ssh[i] for i in fa])
-        K = _prod([ssh[i] for i in self_axes])
-        N = _prod([osh[i] for i in fb])
-        out_shape = [ssh[i] for i in fa] + [osh[i] for i in fb]
+        self_axes, other_axes = tuple(self_axes), tuple(other_axes)
+        key = (ssh, osh, self_axes, other_axes)
+        plan = _CONTRACT_PLANS.get(key)
+        if plan is None:
+            plan = _contract_plan(ssh, osh, self_axes, other_axes)
+            if len(_CONTRACT_PLANS) < 4096:
+                _CONTRACT_PLANS[key] = plan
+        M, N, K, out_shape, a_mode, a_perm, b_mode, b_perm = plan
         out = empty(out_shape)
         if out.numel() == 0:
-            return HipData(out)
+            return HipData._wrap(out)
         if K == 0:
             _lib.call("cc_fill_f64", stream(), out.numel(), 0.0, ptr(out))
-            return HipData(out)
-        # A as (M, K): either free-then-contracted or contracted-then-free in memory order
-        if fa + self_axes == list(range(sn)):
+            return HipData._wrap(out)
+        # A as (M, K): free-then-contracted (0) or contracted-then-free (1) in memory order, else permuted
+        if a_mode == 0:
             A, sAm, sAk = self._t, K, 1
-        elif self_axes + fa == list(range(sn)):
+        elif a_mode == 1:
             A, sAm, sAk = self._t, 1, M
         else:
-            A, sAm, sAk = self._permuted(fa + self_axes), K, 1
-        if other_axes + fb == list(range(on)):
+            A, sAm, sAk = self._permuted(a_perm), K, 1
+        if b_mode == 0:
             B, sBk, sBn = other._t, N, 1
-        elif fb + other_axes == list(range(on)):
+        elif b_mode == 1:
             B, sBk, sBn = other._t, 1, K
         else:
-            B, sBk, sBn = other._permuted(other_axes + fb), N, 1
+            B, sBk, sBn = other._permuted(b_perm), N, 1
         _lib.call("cc_gemm_f64", stream(), 1, M, N, K, 1.0, ptr(A), 0, sAm, sAk, ptr(B), 0, sBk, sBn, 0.0,
                   ptr(out), 0, N, 1)
-        return HipData(out)
+        return HipData._wrap(out)
 
     def contractWithAlongAll(self, other):
         assert self.ndim == other.ndim
