@@ -216,29 +216,33 @@ __global__ void zero_upper_kernel(int64_t n, double* __restrict__ A) {
 }
 
 // Y[i][s] = sum_k M[i][k] X[k][s] for k in [kbeg(i), kend(i)) with M row-major n x n,
-// tri: 0 full, 1 lower (k <= i), 2 upper (k >= i).  One wave per output row.
+// tri: 0 full, 1 lower (k <= i), 2 upper (k >= i).  Two waves per output row, each summing one half of
+// the row's range (half the serial load rounds of one wave per row); the halves are added in order.
 template <int NR>
 __global__ void __launch_bounds__(256) skinny_kernel(int64_t n, int tri, const double* __restrict__ M,
                                                      const double* __restrict__ X, double* __restrict__ Y) {
-  const int lane = threadIdx.x & 63;
-  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (row >= n) return;
-  int64_t kb = (tri == 2) ? row : 0, ke = (tri == 1) ? row + 1 : n;
+  __shared__ double part[4][NR];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, half = wave & 1;
+  const int64_t row = (int64_t)blockIdx.x * 2 + (wave >> 1);
   double acc[NR];
 #pragma unroll
   for (int s = 0; s < NR; ++s) acc[s] = 0.0;
-  const double* mr = M + row * n;
-  int64_t k = kb + lane;
-  for (; k + 192 < ke; k += 256) {  // four independent row loads in flight per lane
-    const double m0 = mr[k], m1 = mr[k + 64], m2 = mr[k + 128], m3 = mr[k + 192];
+  if (row < n) {
+    const int64_t rb = (tri == 2) ? row : 0, re = (tri == 1) ? row + 1 : n, mid = rb + (re - rb) / 2;
+    const int64_t kb = half ? mid : rb, ke = half ? re : mid;
+    const double* mr = M + row * n;
+    int64_t k = kb + lane;
+    for (; k + 192 < ke; k += 256) {  // four independent row loads in flight per lane
+      const double m0 = mr[k], m1 = mr[k + 64], m2 = mr[k + 128], m3 = mr[k + 192];
 #pragma unroll
-    for (int s = 0; s < NR; ++s)
-      acc[s] += m0 * X[k * NR + s] + m1 * X[(k + 64) * NR + s] + m2 * X[(k + 128) * NR + s] + m3 * X[(k + 192) * NR + s];
-  }
-  for (; k < ke; k += 64) {
-    const double m = mr[k];
+      for (int s = 0; s < NR; ++s)
+        acc[s] += m0 * X[k * NR + s] + m1 * X[(k + 64) * NR + s] + m2 * X[(k + 128) * NR + s] + m3 * X[(k + 192) * NR + s];
+    }
+    for (; k < ke; k += 64) {
+      const double m = mr[k];
 #pragma unroll
-    for (int s = 0; s < NR; ++s) acc[s] += m * X[k * NR + s];
+      for (int s = 0; s < NR; ++s) acc[s] += m * X[k * NR + s];
+    }
   }
 #pragma unroll
   for (int s = 0; s < NR; ++s) {
@@ -249,7 +253,12 @@ __global__ void __launch_bounds__(256) skinny_kernel(int64_t n, int tri, const d
   }
   if (lane == 0) {
 #pragma unroll
-    for (int s = 0; s < NR; ++s) Y[row * NR + s] = acc[s];
+    for (int s = 0; s < NR; ++s) part[wave][s] = acc[s];
+  }
+  __syncthreads();
+  if (lane == 0 && half == 0 && row < n) {
+#pragma unroll
+    for (int s = 0; s < NR; ++s) Y[row * NR + s] = part[wave][s] + part[wave + 1][s];
   }
 }
 
@@ -388,7 +397,7 @@ extern "C" int cc_transpose_f64(void* stream, int64_t n, const double* A, double
 
 extern "C" int cc_skinny_f64(void* stream, int64_t n, int tri, const double* M, int nrhs, const double* X, double* Y) {
   if (n <= 0) return CC_OK;
-  unsigned g = (unsigned)((n + 3) / 4);
+  unsigned g = (unsigned)((n + 1) / 2);
   hipStream_t st = cc_stream(stream);
   switch (nrhs) {
     case 1: skinny_kernel<1><<<g, 256, 0, st>>>(n, tri, M, X, Y); break;
