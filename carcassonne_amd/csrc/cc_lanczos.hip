@@ -320,7 +320,7 @@ static int lanczos_driver(hipStream_t st, ApplyFn apply, const void* plan, int64
   set_nan_kernel<<<1, 64, 0, st>>>(res, 2);
   CC_LAUNCH_CHECK();
   double prev_theta = NAN, theta = NAN, resid = NAN;
-  int m = 0;
+  int m = 0, checked_m = -1;
   for (int j = 0; j < max_iter; ++j) {
     double* qj = Q + (int64_t)j * n;
     rc = op_whitened(st, apply, plan, wh, qj, t1, t2, w);
@@ -338,23 +338,17 @@ static int lanczos_driver(hipStream_t st, ApplyFn apply, const void* plan, int64
     m = j + 1;
     if (m == n) break;
     if (!fixed && (m % check_every == 0 || m == max_iter)) {
-      // Ritz value every check; the residual estimate (needs the Ritz vector) once the value has settled
-      bool settle_check = false;
-      for (int pass = 0; pass < 2; ++pass) {
-        const int want = pass == 1 ? 1 : 0;
-        lz_tridiag_kernel<<<1, 64, 0, st>>>(m, alpha, beta, res, s, scratch, want);
-        CC_LAUNCH_CHECK();
-        double h[2];
-        CC_HIP(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, st));
-        CC_HIP(hipStreamSynchronize(st));
-        theta = h[0];
-        if (pass == 0) {
-          settle_check = !std::isnan(prev_theta) && prev_theta - theta <= 1e3 * tol_energy * fabs(theta);
-          if (!settle_check && !(m == max_iter || m == n)) break;
-        } else {
-          resid = h[1];
-        }
-      }
+      // Ritz value, Ritz vector and residual estimate in one pass (one read of two doubles); the
+      // residual counts once the value has settled
+      lz_tridiag_kernel<<<1, 64, 0, st>>>(m, alpha, beta, res, s, scratch, 1);
+      CC_LAUNCH_CHECK();
+      double h[2];
+      CC_HIP(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, st));
+      CC_HIP(hipStreamSynchronize(st));
+      theta = h[0];
+      resid = h[1];
+      checked_m = m;
+      const bool settle_check = !std::isnan(prev_theta) && prev_theta - theta <= 1e3 * tol_energy * fabs(theta);
       bool done = m == n || m == max_iter ||
                   (settle_check && (resid <= tol_residual * fabs(theta) ||
                                     (prev_theta - theta <= tol_energy * fabs(theta))));
@@ -362,15 +356,17 @@ static int lanczos_driver(hipStream_t st, ApplyFn apply, const void* plan, int64
       if (done) break;
     }
   }
-  // the final Ritz vector of T_m
-  lz_tridiag_kernel<<<1, 64, 0, st>>>(m, alpha, beta, res, s, scratch, 1);
-  CC_LAUNCH_CHECK();
-  if (!fixed) {
-    double h[2];
-    CC_HIP(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, st));
-    CC_HIP(hipStreamSynchronize(st));
-    theta = h[0];
-    resid = h[1];
+  // the final Ritz vector of T_m (already in s when the last check was at this m)
+  if (checked_m != m) {
+    lz_tridiag_kernel<<<1, 64, 0, st>>>(m, alpha, beta, res, s, scratch, 1);
+    CC_LAUNCH_CHECK();
+    if (!fixed) {
+      double h[2];
+      CC_HIP(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, st));
+      CC_HIP(hipStreamSynchronize(st));
+      theta = h[0];
+      resid = h[1];
+    }
   }
   // Ritz vector z = Q s, x = W^T z   (N-normalised because ||z|| = 1)
   lz_combine_kernel<<<(unsigned)((n + 15) / 16), 256, 0, st>>>(m, n, Q, s, t2, 1);
