@@ -371,9 +371,8 @@ static int lanczos_driver(hipStream_t st, ApplyFn apply, const void* plan, int64
   // Ritz vector z = Q s, x = W^T z   (N-normalised because ||z|| = 1)
   lz_combine_kernel<<<(unsigned)((n + 15) / 16), 256, 0, st>>>(m, n, Q, s, t2, 1);
   CC_LAUNCH_CHECK();
-  rc = cc_skinny_f64(st, n_env, 2, Wt, (int)d, t2, x_out);
+  rc = cc_skinny_f64(st, n_env, 2, Wt, (int)d, t2, x_out);  // stream-ordered: no wait for it here
   if (rc) return rc;
-  if (!fixed) CC_HIP(hipStreamSynchronize(st));
   if (host_result) {
     host_result[0] = theta;   // NaN in the fixed mode: the Ritz value and residual stay on the device
     host_result[1] = resid;

@@ -174,7 +174,8 @@ int cc_skinny_f64(void* stream, int64_t n, int tri, const double* M, int nrhs, c
  * N = (L L^T) (x) I_d, on the whitened operator W H W^T (W = L^-1).  H is a stage-3 plan.
  * Stops when the Ritz residual <= tol_residual*|theta|, when the Ritz value moved by at most
  * tol_energy*|theta| since the previous check (every check_every steps), or at max_iter.
- * x_out is N-normalised; host_result = {theta, residual estimate, iterations}.  check_every = 0 runs
+ * x_out is N-normalised and written in stream order (the call does not wait for it); host_result =
+ * {theta, residual estimate, iterations}.  check_every = 0 runs
  * exactly max_iter steps (fewer if the Krylov space completes) without any host read or
  * synchronisation, so the call can be captured in a HIP graph; host_result (may be null) then receives
  * NaN for theta and the residual, which stay in device memory.
