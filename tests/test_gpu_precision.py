@@ -40,7 +40,7 @@ def test_lowprec_matvec_close_to_fp64(gpu, dtype, tol, D, chi, split):
 
 @pytest.mark.parametrize("D,chi", [(4, 16), (6, 12)])
 def test_lowprec_matvec_any_alpha_grouping(gpu, D, chi):
-    """The FP16 kernel pipelines the alphas of each workgroup three deep; workgroups of 1, 2, 3, 4, 5 and
+    """The FP16 kernel double-buffers the alphas of each workgroup; workgroups of 1, 2, 3, 4, 5 and
     uneven alpha counts (and more workgroups than alphas) give the FP64 result to the FP16 tolerance and
     the default grouping's result up to FP32 summation order."""
     from carcassonne_amd.precision import LowPrecisionOperator

@@ -148,3 +148,34 @@ def test_non_positive_normalization_is_reported_by_the_solve(gpu):
     negated = stage3.Stage3Operator([(s0 * -1.0, s1, None)], physical_dimension=N.operator.d)
     with pytest.raises(ValueError, match="not positive definite"):
         GroundStateSolver(H.operator, negated).solve(torch.from_numpy(st.ravel()).cuda())
+
+
+def test_stage12_graph_replays_match_eager(gpu):
+    """The replayed stage-1/2 graph gives the eager stage 1/2 on every replay: two environments of one
+    structure alternate (A, B, A, B) with eager work between the replays, and each replay's H and N
+    products on a seeded vector, and its mixed-precision energy, equal the eager path's."""
+    from carcassonne_amd import pipeline, synthetic as syn
+    from carcassonne_amd.data import HipData
+    envs = []
+    for seed in (3, 4):
+        c, s, st = syn.make_environment(4, 16, model="ising", seed=seed)
+        envs.append((pipeline.device_environment(c, s, syn.operator_center("ising", h=2.5 + seed / 10)), st))
+    pipeline._STAGE12.clear()
+    rng = np.random.default_rng(0)
+    for rep, i in enumerate((0, 1, 0, 1)):
+        (corners, sides, opc), st = envs[i]
+        Hg, Ng = pipeline.multipliers(corners, sides, opc, graph=True)
+        # eager work between replays, then the eager reference
+        He, Ne = pipeline.multipliers(corners, sides, opc, graph=False)
+        torch.cuda.synchronize()
+        x = torch.from_numpy(rng.standard_normal(Hg.operator.n)).cuda()
+        for g, e in ((Hg.operator, He.operator), (Ng.operator, Ne.operator)):
+            yg, ye = torch.empty_like(x), torch.empty_like(x)
+            g.apply_flat(x, yg)
+            e.apply_flat(x, ye)
+            assert torch.allclose(yg, ye, rtol=1e-12, atol=1e-12 * float(ye.abs().max())), (rep, i)
+        x0 = HipData.fromArray(st)
+        rg = pipeline.solve_mixed(corners, sides, opc, x0, graph=True)
+        re = pipeline.solve_mixed(corners, sides, opc, x0, graph=False)
+        assert abs(rg["energy"] - re["energy"]) <= 1e-11 * abs(re["energy"]), (rep, i, rg["energy"], re["energy"])
+    assert len(pipeline._STAGE12) == 1
