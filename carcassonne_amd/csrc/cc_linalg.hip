@@ -57,15 +57,17 @@ __device__ __forceinline__ void tri_sweep(double (&r)[CB], double s, F src) {
 // c[j] = -d_i X[j][i] (j > i).  The symmetric updates keep c_i[j] == c_j[i] bit for bit (the same two
 // factors meet in each FMA).  A non-positive pivot of a real row records its 1-based index in bad and is
 // replaced by 1e-300.
-// Four waves: wave w holds columns 16w .. 16w + 15 of every row.  At step K the wave owning column K
-// forms the pivot and the multipliers, publishes them in LDS (two buffers by step parity, so one barrier
-// per step), and every wave with columns past K updates them.
-constexpr int CW = 16, NW4 = CB / CW;
-// Pivot of step K in its owner wave K / CW: the multipliers s_i into sb and c[K % CW] set.
-template <int K>
-__device__ __forceinline__ void chol_pivot4(double (&c)[CW], double* sb, int i, int kb, int& bad) {
+// Blocks of 64R x 64R on 4R waves: wave w holds columns 16w .. 16w + 15, lane i rows i + 64r (r < R).
+// At step K the wave owning column K forms the pivot and the multipliers, publishes them in LDS (two
+// buffers by step parity, so one barrier per step), and every wave with columns past K updates them.
+// Steps at or past the block's kb real rows only meet the identity padding and change nothing, so the
+// sweep ends there.
+constexpr int CW = 16;
+// Pivot of step K in its owner wave K / CW: the multipliers s_row into sb and c[.][K % CW] set.
+template <int K, int R>
+__device__ __forceinline__ void chol_pivot4(double (&c)[R][CW], double* sb, int i, int kb, int& bad) {
   constexpr int k = K % CW;
-  double d = readlane_f64(c[k], K);
+  double d = readlane_f64(c[K / 64][k], K % 64);
   if (!(d > 0.0)) {
     if (K < kb && bad == 0) bad = K + 1;
     d = 1e-300;
@@ -77,86 +79,117 @@ __device__ __forceinline__ void chol_pivot4(double (&c)[CW], double* sb, int i, 
 #pragma unroll
   for (int it = 0; it < 2; ++it) inv = fma(0.5 * inv, fma(-d * inv, inv, 1.0), inv);
   const double sd = d * inv;
-  const double sv = (i == K) ? 0.0 : c[k] * inv;
-  c[k] = (i == K) ? sd : sv;
-  sb[i] = sv;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int row = i + 64 * r;
+    const double sv = (row == K) ? 0.0 : c[r][k] * inv;
+    c[r][k] = (row == K) ? sd : sv;
+    sb[row] = sv;
+  }
 }
 
 // Step K (its multipliers already in sbuf[K & 1]): after the barrier the owner of column K + 1 applies
 // step K to that column first and forms the pivot of step K + 1 into the other buffer (look-ahead), so
 // the pivot chain overlaps the other columns' updates.
-template <int K>
-__device__ __forceinline__ void chol_inv_steps4(double (&c)[CW], double (*sbuf)[CB], int w, int i, int kb, int& bad) {
-  if constexpr (K < CB) {
+template <int K, int R>
+__device__ __forceinline__ void chol_inv_steps4(double (&c)[R][CW], double (*sbuf)[64 * R], int w, int i, int kb, int& bad) {
+  if constexpr (K < 64 * R) {
+    if (K >= kb) return;
     constexpr int o = K / CW, k = K % CW;
-    constexpr bool ahead = K + 1 < CB;
+    constexpr bool ahead = K + 1 < 64 * R;
     constexpr int o1 = ahead ? (K + 1) / CW : 0, k1 = ahead ? (K + 1) % CW : 0;
     __syncthreads();
     const double* sb = sbuf[K & 1];
     if (w >= o) {
-      const double si = sb[i];
+      double si[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) si[r] = sb[i + 64 * r];
       if (ahead && w == o1) {
-        c[k1] -= si * sb[K + 1];
-        if constexpr (ahead) chol_pivot4<K + 1>(c, sbuf[(K + 1) & 1], i, kb, bad);
+#pragma unroll
+        for (int r = 0; r < R; ++r) c[r][k1] -= si[r] * sb[K + 1];
+        if constexpr (ahead) chol_pivot4<K + 1, R>(c, sbuf[(K + 1) & 1], i, kb, bad);
       }
       if (w == o) {
         // columns past K in the owner wave, less column K + 1 when the look-ahead took it
         constexpr int j0 = (ahead && o1 == o) ? k + 2 : k + 1;
 #pragma unroll
-        for (int jj = j0; jj < CW; ++jj) c[jj] -= si * sb[CW * o + jj];
-      } else if (ahead && o1 != o && w == o1) {
-        const double* sw = sb + CW * w;
+        for (int jj = j0; jj < CW; ++jj) {
+          const double sj = sb[CW * o + jj];
 #pragma unroll
-        for (int jj = 1; jj < CW; ++jj) c[jj] -= si * sw[jj];
+          for (int r = 0; r < R; ++r) c[r][jj] -= si[r] * sj;
+        }
       } else {
         const double* sw = sb + CW * w;
+        const int j0 = (ahead && o1 != o && w == o1) ? 1 : 0;  // column 0 taken by the look-ahead
 #pragma unroll
-        for (int jj = 0; jj < CW; ++jj) c[jj] -= si * sw[jj];
+        for (int jj = 0; jj < CW; ++jj)
+          if (jj >= j0) {
+            const double sj = sw[jj];
+#pragma unroll
+            for (int r = 0; r < R; ++r) c[r][jj] -= si[r] * sj;
+          }
       }
     }
-    chol_inv_steps4<K + 1>(c, sbuf, w, i, kb, bad);
+    chol_inv_steps4<K + 1, R>(c, sbuf, w, i, kb, bad);
   }
 }
 
-// Cholesky and inverse of one diagonal block of at most 64 x 64 (see chol_inv_steps4): A's block at
+// Cholesky and inverse of one diagonal block of at most 64R x 64R (see chol_inv_steps4): A's block at
 // (k0, k0) (lower triangle read; rows >= kb padded with the identity) is replaced by L with its upper
 // triangle zeroed, X's block at (x0, x0) receives L^-1, info the first non-positive pivot (1-based,
 // global index).
-__global__ void __launch_bounds__(64 * NW4) chol_leaf_kernel(int64_t lda, int64_t k0, int kb, double* __restrict__ A,
-                                                           double* __restrict__ X, int64_t ldx, int64_t x0,
-                                                           int* __restrict__ info) {
-  __shared__ double Ls[CB][CB + 1];
-  __shared__ double sbuf[2][CB];
+template <int R>
+__global__ void __launch_bounds__(256 * R) chol_leaf_kernel(int64_t lda, int64_t k0, int kb, double* __restrict__ A,
+                                                         double* __restrict__ X, int64_t ldx, int64_t x0,
+                                                         int* __restrict__ info) {
+  constexpr int BN = 64 * R, NW = BN / CW;
+  __shared__ double Ls[BN][BN + 1];
+  __shared__ double sbuf[2][BN];
+  __shared__ int badw[NW];
   const int i = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int ic = i < kb ? i : 0;  // clamped: every load below is in bounds
-  double c[CW];
-  // row i of the lower triangle, then column i of it mirrored into j > i (identity past kb)
+  double c[R][CW];
+  // row i + 64r of the lower triangle, then its column mirrored into j > row (identity past kb); every
+  // load index clamped into the block
 #pragma unroll
-  for (int jj = 0; jj < CW; ++jj) {
-    const int j = CW * w + jj, jc = j < kb ? j : 0;
-    const double v = A[(k0 + (j <= i ? ic : jc)) * lda + k0 + (j <= i ? jc : ic)];
-    c[jj] = (i < kb && j < kb) ? v : (i == j ? 1.0 : 0.0);
+  for (int r = 0; r < R; ++r) {
+    const int row = i + 64 * r, rc = row < kb ? row : 0;
+#pragma unroll
+    for (int jj = 0; jj < CW; ++jj) {
+      const int j = CW * w + jj, jc = j < kb ? j : 0;
+      const double v = A[(k0 + (j <= row ? rc : jc)) * lda + k0 + (j <= row ? jc : rc)];
+      c[r][jj] = (row < kb && j < kb) ? v : (row == j ? 1.0 : 0.0);
+    }
   }
   int bad = 0;  // first failing pivot among this wave's columns
-  if (w == 0) chol_pivot4<0>(c, sbuf[0], i, kb, bad);
-  chol_inv_steps4<0>(c, sbuf, w, i, kb, bad);
-  __shared__ int badw[NW4];
+  if (w == 0) chol_pivot4<0, R>(c, sbuf[0], i, kb, bad);
+  chol_inv_steps4<0, R>(c, sbuf, w, i, kb, bad);
   if (i == 0) badw[w] = bad;
 #pragma unroll
-  for (int jj = 0; jj < CW; ++jj) Ls[i][CW * w + jj] = c[jj];
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int jj = 0; jj < CW; ++jj) Ls[i + 64 * r][CW * w + jj] = c[r][jj];
   __syncthreads();
   if (threadIdx.x == 0 && *info == 0)
-    for (int v = 0; v < NW4; ++v)
+    for (int v = 0; v < NW; ++v)
       if (badw[v]) {
         *info = (int)(k0 + badw[v]);
         break;
       }
-  for (int rr = w; rr < kb; rr += NW4)
-    if (i < kb) A[(k0 + rr) * lda + k0 + i] = (i <= rr) ? Ls[rr][i] : 0.0;
-  // X[m][i] = -c_i[m] / d_i below the diagonal, 1 / L[i][i] on it
-  const double xd = 1.0 / Ls[i][i], rd = -xd * xd;
-  for (int m = w; m < kb; m += NW4)
-    if (i < kb) X[(x0 + m) * ldx + x0 + i] = (m > i) ? Ls[i][m] * rd : (m == i ? xd : 0.0);
+  for (int rr = w; rr < kb; rr += NW)
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int col = i + 64 * r;
+      if (col < kb) A[(k0 + rr) * lda + k0 + col] = (col <= rr) ? Ls[rr][col] : 0.0;
+    }
+  // X[m][row] = -c_row[m] / d_row below the diagonal, 1 / L[row][row] on it
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int row = i + 64 * r;
+    if (row < kb) {
+      const double xd = 1.0 / Ls[row][row], rd = -xd * xd;
+      for (int m = w; m < kb; m += NW) X[(x0 + m) * ldx + x0 + row] = (m > row) ? Ls[row][m] * rd : (m == row ? xd : 0.0);
+    }
+  }
 }
 
 // Column sweep of L x = e_j, steps Q..CB-1: x[Q] *= 1/L[Q][Q], then x[m] -= L[m][Q] x[Q] for m > Q.
@@ -276,15 +309,17 @@ inline int64_t split_rows(int64_t sz) { return ((sz + CB - 1) / CB / 2) * CB; }
 //   (L11, X11) = chol_inv(A11); L21 = A21 X11^T; A22 -= L21 L21^T; (L22, X22) = chol_inv(A22);
 //   X21 = -X22 (L21 X11).
 // The GEMMs have the full K of each level (not the block width), so they fill the chip; leaves are
-// chol_leaf_kernel.  tmp holds n1 x n2 <= sz^2 / 4 doubles.
+// chol_leaf_kernel (blocks of up to 128 rows, split at multiples of 128).  tmp holds n1 x n2 <= sz^2 / 4
+// doubles.
 int chol_inv_rec(void* stream, int64_t n, int64_t off, int64_t sz, double* A, double* X, double* tmp, int* info) {
   hipStream_t st = cc_stream(stream);
-  if (sz <= CB) {
-    chol_leaf_kernel<<<1, 64 * NW4, 0, st>>>(n, off, (int)sz, A, X, n, off, info);
+  if (sz <= 2 * CB) {
+    if (sz <= CB) chol_leaf_kernel<1><<<1, 256, 0, st>>>(n, off, (int)sz, A, X, n, off, info);
+    else chol_leaf_kernel<2><<<1, 512, 0, st>>>(n, off, (int)sz, A, X, n, off, info);
     CC_LAUNCH_CHECK();
     return CC_OK;
   }
-  const int64_t n1 = split_rows(sz), n2 = sz - n1;
+  const int64_t n1 = ((sz + 2 * CB - 1) / (2 * CB) / 2) * (2 * CB), n2 = sz - n1;
   int rc = chol_inv_rec(stream, n, off, n1, A, X, tmp, info);
   if (rc) return rc;
   double* A21 = A + (off + n1) * n + off;
@@ -345,7 +380,7 @@ extern "C" int cc_cholesky_f64(void* stream, int64_t n, double* A, int* info_dev
   CC_LAUNCH_CHECK();
   for (int64_t k0 = 0; k0 < n; k0 += CB) {
     int kb = (int)((n - k0) < CB ? (n - k0) : CB);
-    chol_leaf_kernel<<<1, 64 * NW4, 0, st>>>(n, k0, kb, A, dinv, kb, 0, info_dev);
+    chol_leaf_kernel<1><<<1, 256, 0, st>>>(n, k0, kb, A, dinv, kb, 0, info_dev);
     CC_LAUNCH_CHECK();
     int64_t rest = n - k0 - kb;
     if (rest > 0) {
