@@ -76,9 +76,12 @@ class Stage12Graph:
 
     def run(self, corners, sides):
         sc, ss = self._static
+        dsts, srcs = [], []
         for dst, src in zip(sc + ss, list(corners) + list(sides)):
             for k, v in src.items():
-                dst[k]._t.copy_(v._t)
+                dsts.append(dst[k]._t)
+                srcs.append(v._t)
+        torch._foreach_copy_(dsts, srcs)   # one multi-tensor launch for the whole environment
         self.graph.replay()
         for b in self.stage2:
             for t in b.values():
