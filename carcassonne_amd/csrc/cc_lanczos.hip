@@ -35,10 +35,12 @@ __device__ inline double block_dot256(int64_t n, const double* __restrict__ a, c
   return red[0];
 }
 
+// c[b] = Q_b . w for b < nq; a block b == nq (when launched) gives c[nq] = w . w
 __global__ void __launch_bounds__(256) lz_coeff_kernel(int64_t n, const double* __restrict__ Q, const double* __restrict__ w,
-                                                       double* __restrict__ c) {
+                                                       double* __restrict__ c, int nq) {
   __shared__ double red[256];
-  const double t = block_dot256(n, Q + (int64_t)blockIdx.x * n, w, red);
+  const double* q = (int)blockIdx.x == nq ? w : Q + (int64_t)blockIdx.x * n;
+  const double t = block_dot256(n, q, w, red);
   if (threadIdx.x == 0) c[blockIdx.x] = t;
 }
 
@@ -63,6 +65,32 @@ __global__ void __launch_bounds__(256) lz_combine_kernel(int m, int64_t n, const
     for (int k = 0; k < 16; ++k) t += red[k][e];
     w[i] = assign ? t : w[i] - t;
   }
+}
+
+// The second Gram-Schmidt pass and the normalisation in one kernel: q = (w - Q c) / beta with
+// beta^2 = w.w - c.c (c[m] holds w.w from lz_coeff_kernel; exact for orthonormal Q, and free of
+// cancellation because the second pass's coefficients are at rounding level), beta written by block 0.
+__global__ void __launch_bounds__(256) lz_combine_normalize_kernel(int m, int64_t n, const double* __restrict__ Q,
+                                                                   const double* __restrict__ c, const double* __restrict__ w,
+                                                                   double* __restrict__ q, double* __restrict__ beta) {
+  __shared__ double red[16][16];
+  const int e = threadIdx.x & 15, slice = threadIdx.x >> 4;
+  const int64_t i = (int64_t)blockIdx.x * 16 + e;
+  double acc = 0.0;
+  if (i < n)
+    for (int j = slice; j < m; j += 16) acc += c[j] * Q[(int64_t)j * n + i];
+  red[slice][e] = acc;
+  double cc = 0.0;
+  for (int j = 0; j < m; ++j) cc += c[j] * c[j];
+  const double b2 = c[m] - cc, b = b2 > 0.0 ? sqrt(b2) : 0.0, inv = b > 0.0 ? 1.0 / b : 0.0;
+  __syncthreads();
+  if (slice == 0 && i < n) {
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) t += red[k][e];
+    q[i] = (w[i] - t) * inv;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *beta = b;
 }
 
 // w -= alpha q_j + beta q_{j-1}  (alpha, beta read from device memory)
@@ -327,13 +355,15 @@ static int lanczos_driver(hipStream_t st, ApplyFn apply, const void* plan, int64
     if (rc) return rc;
     // full reorthogonalisation, two classical Gram-Schmidt passes against q_0..q_j; the first pass's
     // coefficient on q_j is alpha_j = q_j . A q_j (the three-term recurrence is contained in it)
-    for (int pass = 0; pass < 2; ++pass) {
-      lz_coeff_kernel<<<j + 1, 256, 0, st>>>(n, Q, w, c);
-      CC_LAUNCH_CHECK();
-      lz_combine_kernel<<<(unsigned)((n + 15) / 16), 256, 0, st>>>(j + 1, n, Q, c, w, 0, pass == 0 ? alpha + j : nullptr);
-      CC_LAUNCH_CHECK();
-    }
-    lz_normalize_kernel<<<1, 1024, 0, st>>>(n, w, Q + (int64_t)(j + 1) * n, beta + j);
+    lz_coeff_kernel<<<j + 1, 256, 0, st>>>(n, Q, w, c, -1);
+    CC_LAUNCH_CHECK();
+    lz_combine_kernel<<<(unsigned)((n + 15) / 16), 256, 0, st>>>(j + 1, n, Q, c, w, 0, alpha + j);
+    CC_LAUNCH_CHECK();
+    // second pass with the normalisation folded in (w . w from one more coefficient block)
+    lz_coeff_kernel<<<j + 2, 256, 0, st>>>(n, Q, w, c, j + 1);
+    CC_LAUNCH_CHECK();
+    lz_combine_normalize_kernel<<<(unsigned)((n + 15) / 16), 256, 0, st>>>(j + 1, n, Q, c, w, Q + (int64_t)(j + 1) * n,
+                                                                           beta + j);
     CC_LAUNCH_CHECK();
     m = j + 1;
     if (m == n) break;
