@@ -47,6 +47,7 @@ SIGNATURES = {
     "cc_stage3_plan_apply": [_vp, _vp, _vp, _vp],
     "cc_stage3_plan_destroy": [_vp],
     "cc_profile_begin": [],
+    "cc_profile_begin_sampled": [_i32],
     "cc_profile_end": [_i32, _pf64, _pi64],
     "cc_linalg_work_size": [_i64, _pi64],
     "cc_cholesky_f64": [_vp, _i64, _vp, _vp, _vp],

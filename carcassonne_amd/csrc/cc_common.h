@@ -45,10 +45,11 @@ int64_t cc_internal_stage3_n(const void* plan);
 int cc_internal_lp_apply(hipStream_t st, const void* plan, const double* x, double* y);
 int64_t cc_internal_lp_n(const void* plan);
 
-// Launch timing (between cc_profile_begin and cc_profile_end, single-threaded use): a kernel launch is
-// bracketed by cc_prof_start / cc_prof_stop, which record HIP events on its stream when timing is on.
+// Launch timing (between cc_profile_begin[_sampled] and cc_profile_end, single-threaded use): a kernel
+// launch is bracketed by cc_prof_start / cc_prof_stop, which record HIP events on its stream when timing
+// is on and the launch is one of the sampled ones of its kind (e0 stays null otherwise).
 enum { CC_PROF_FUSED_F64 = 0, CC_PROF_LP = 1, CC_PROF_KINDS = 2 };
-int cc_prof_start(hipStream_t st, hipEvent_t* e0);
+int cc_prof_start(hipStream_t st, int kind, hipEvent_t* e0);
 int cc_prof_stop(hipStream_t st, int kind, hipEvent_t e0);
 
 // A stage-3 slab channel read in place from a tensor with six index legs: slab[alpha][p'][p] with

@@ -264,26 +264,43 @@ extern "C" int cc_scale_rows_cols_f64(void* stream, int cols, int64_t m, int64_t
 }
 
 // ---------------------------------------------------------------- launch timing
+// Events are recorded with a device-scope release (no system-scope cache write-back around the timed
+// kernel, which would idle the GPU between the kernel and its neighbours) and come from a pool that
+// lives until the process ends, so a timed launch costs two event records and no event creation.
 namespace {
 bool g_profile = false;
+int g_profile_every = 1;
+int64_t g_profile_seen[CC_PROF_KINDS];
 struct ProfRecord {
   int kind;
   hipEvent_t e0, e1;
 };
 std::vector<ProfRecord> g_prof;
+std::vector<hipEvent_t> g_event_pool;
 void prof_clear() {
   for (const ProfRecord& r : g_prof) {
-    (void)hipEventDestroy(r.e0);
-    (void)hipEventDestroy(r.e1);
+    g_event_pool.push_back(r.e0);
+    g_event_pool.push_back(r.e1);
   }
   g_prof.clear();
 }
+int prof_event(hipEvent_t* e) {
+  if (!g_event_pool.empty()) {
+    *e = g_event_pool.back();
+    g_event_pool.pop_back();
+    return CC_OK;
+  }
+  CC_HIP(hipEventCreateWithFlags(e, hipEventReleaseToDevice));
+  return CC_OK;
+}
 }  // namespace
 
-int cc_prof_start(hipStream_t st, hipEvent_t* e0) {
+int cc_prof_start(hipStream_t st, int kind, hipEvent_t* e0) {
   *e0 = nullptr;
-  if (!g_profile) return CC_OK;
-  CC_HIP(hipEventCreate(e0));
+  if (!g_profile || kind < 0 || kind >= CC_PROF_KINDS) return CC_OK;
+  if (g_profile_seen[kind]++ % g_profile_every != 0) return CC_OK;
+  const int rc = prof_event(e0);
+  if (rc != CC_OK) return rc;
   CC_HIP(hipEventRecord(*e0, st));
   return CC_OK;
 }
@@ -291,17 +308,26 @@ int cc_prof_start(hipStream_t st, hipEvent_t* e0) {
 int cc_prof_stop(hipStream_t st, int kind, hipEvent_t e0) {
   if (!g_profile || !e0) return CC_OK;
   hipEvent_t e1 = nullptr;
-  CC_HIP(hipEventCreate(&e1));
+  const int rc = prof_event(&e1);
+  if (rc != CC_OK) return rc;
   CC_HIP(hipEventRecord(e1, st));
   g_prof.push_back(ProfRecord{kind, e0, e1});
   return CC_OK;
 }
 
-extern "C" int cc_profile_begin(void) {
+extern "C" int cc_profile_begin_sampled(int every) {
+  if (every < 1) {
+    cc_set_last_error("cc_profile_begin_sampled: every must be at least 1");
+    return CC_ERR_ARG;
+  }
   prof_clear();
   g_profile = true;
+  g_profile_every = every;
+  for (int k = 0; k < CC_PROF_KINDS; ++k) g_profile_seen[k] = 0;
   return CC_OK;
 }
+
+extern "C" int cc_profile_begin(void) { return cc_profile_begin_sampled(1); }
 
 extern "C" int cc_profile_end(int n_kinds, double* total_ms, int64_t* launches) {
   g_profile = false;

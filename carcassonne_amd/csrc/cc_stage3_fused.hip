@@ -213,7 +213,7 @@ __global__ void __launch_bounds__(256) reduce_partials_kernel(int64_t groups, in
 template <int P, int DP>
 int launch_fused(hipStream_t st, const FusedPlan& plan, const double* v, double* y, double* work, int groups) {
   hipEvent_t e0 = nullptr;
-  int rc = cc_prof_start(st, &e0);
+  int rc = cc_prof_start(st, CC_PROF_FUSED_F64, &e0);
   if (rc != CC_OK) return rc;
   fused_f64_kernel<P, DP><<<groups, 256, 0, st>>>(plan, v, work);
   CC_LAUNCH_CHECK();

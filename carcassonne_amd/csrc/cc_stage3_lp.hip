@@ -552,7 +552,7 @@ int launch_lp(hipStream_t st, const LPHandle* h, const double* v, double* y) {
   double* scale = reinterpret_cast<double*>(h->work + L.scale);
   const dim3 grid((unsigned)h->groups, (unsigned)h->chunks);
   hipEvent_t e0 = nullptr;
-  int rc = cc_prof_start(st, &e0);
+  int rc = cc_prof_start(st, CC_PROF_LP, &e0);
   if (rc != CC_OK) return rc;
   if (h->split == LP_SPLIT_BOTH) lp_kernel<P, DP, T, LP_SPLIT_BOTH><<<grid, 64 * NT, 0, st>>>(plan, v, scale, partial);
   else if (h->split == LP_SPLIT_VECTOR) lp_kernel<P, DP, T, LP_SPLIT_VECTOR><<<grid, 64 * NT, 0, st>>>(plan, v, scale, partial);

@@ -148,9 +148,12 @@ int cc_stage3_plan_destroy(void* plan);
 
 /* Timing of the stage-3 kernels: between begin and end every launch of the FP64 fused kernel (kind 0)
  * and of the low-precision kernel (kind 1) is bracketed by HIP events on its stream; end fills
- * total_ms[k] and launches[k] for k < n_kinds.  Single-threaded use only.  (Measurement support; no
- * reference counterpart.) */
+ * total_ms[k] and launches[k] (the launches timed) for k < n_kinds.  begin_sampled(every) times only
+ * launches 0, every, 2 every, ... of each kind, so that the event records (each idles the GPU for a few
+ * microseconds) cost a fraction of the timed work.  Single-threaded use only.  (Measurement support;
+ * no reference counterpart.) */
 int cc_profile_begin(void);
+int cc_profile_begin_sampled(int every);
 int cc_profile_end(int n_kinds, double* total_ms, int64_t* launches);
 
 /* ---- dense linear algebra of the solve --------------------------------------------

@@ -69,3 +69,25 @@ def test_mixed_solve_reaches_fp64_energy(gpu, dtype, key):
     err = abs(rho - g["ground_energy"]) / abs(g["ground_energy"])
     print(dtype, key, "err", err, info)
     assert err < 1e-10, (dtype, key, rho, g["ground_energy"], err, info)
+
+
+def test_sampled_launch_timing_counts(gpu):
+    """cc_profile_begin_sampled(k) times launches 0, k, 2k, ... of each kind: 10 low-precision applies at
+    k = 4 give 3 timed launches with a positive duration, and no FP64 launches."""
+    from carcassonne_amd import _lib
+    from carcassonne_amd.precision import LowPrecisionOperator
+    H, N, st = _operators(4, 16, "ising", 7)
+    x = torch.from_numpy(st.ravel()).cuda()
+    y = torch.empty_like(x)
+    lp = LowPrecisionOperator(H, "f16")
+    lp.apply_flat(x, y)
+    torch.cuda.synchronize()
+    _lib.call("cc_profile_begin_sampled", 4)
+    for _ in range(10):
+        lp.apply_flat(x, y)
+    kms, launches = _lib.f64_array([0, 0]), _lib.i64_array([0, 0])
+    _lib.call("cc_profile_end", 2, kms, launches)
+    assert list(launches) == [0, 3]
+    assert kms[1] > 0.0
+    with pytest.raises(Exception):
+        _lib.call("cc_profile_begin_sampled", 0)
