@@ -64,7 +64,6 @@ struct LPDevPlan {
   int n_chunks, pad;
   LPChunk chunk[LP_MAX_CHUNKS];
   int term_slot[LP_MAX_TERMS];     // LDS slot of each (chunk-ordered) term's B slab
-  int term_diag[LP_MAX_TERMS];     // 1 when the term's coefficient matrix is diagonal
   int term_a[LP_MAX_TERMS];        // the term's A and B channel indices (input order = chunk order)
   int term_b[LP_MAX_TERMS];
   float coef[LP_MAX_TERMS][LP_MAX_D * LP_MAX_D];
@@ -152,22 +151,76 @@ __device__ __attribute__((always_inline)) inline void lp_stage(const LPChunk* __
   }
 }
 
+// The chunk's term and run tables, one entry per lane (lane l: chunk-ordered term l, run l), read with
+// v_readlane inside the loops so that no scalar memory load sits on the per-term path.
+template <int DP>
+struct LPMeta {
+  int tslot;            // term l's B slot
+  float coef[DP * DP];  // term l's coefficient matrix C_t[s'][s] at s' DP + s (diagonal terms too)
+  int rslot, rt0;       // run l's A slot and first term (rt0 of lane n_runs: the chunk's term count)
+};
+template <int DP>
+__device__ inline LPMeta<DP> lp_meta(const LPDevPlan* __restrict__ plan, const LPChunk* __restrict__ C, int lane) {
+  LPMeta<DP> m;
+  const int t = lane < LP_MAX_TERMS ? lane : LP_MAX_TERMS - 1;
+  m.tslot = plan->term_slot[t];
+#pragma unroll
+  for (int k = 0; k < DP * DP; ++k) m.coef[k] = plan->coef[t][k];
+  m.rslot = C->run_slot[t];
+  m.rt0 = C->run_t0[lane <= LP_MAX_TERMS ? lane : LP_MAX_TERMS];
+  return m;
+}
+__device__ inline int lp_lane_int(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ inline float lp_lane_float(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+
+// Y[s'] += sum_s C[s'][s] Z[s] for one term's products
+template <int NT, int DP>
+__device__ __forceinline__ void lp_combine(f32x4 (&y)[NT][DP], const f32x4 (&z)[NT][DP], const float (&c)[DP * DP]) {
+#pragma unroll
+  for (int nq = 0; nq < NT; ++nq)
+#pragma unroll
+    for (int sp = 0; sp < DP; ++sp)
+#pragma unroll
+      for (int s = 0; s < DP; ++s) y[nq][sp] += c[sp * DP + s] * z[nq][s];
+}
+
 // One alpha of one chunk for wave np (p' tile np).  Stage A: W tiles m = s*QT + mq over K tiles of p,
 // converted once per A run to hi + lo stage-B operands (element e < 4 from W tile s*QT + 2t, e >= 4 from
 // tile s*QT + 2t + 1, zero past QT).  Per term: Z[s] = W[s] B over K tiles of q, then
-// Y[s'] += sum_s C_t[s'][s] Z[s] (only the diagonal when C_t is diagonal), which keeps the per-term
-// vector work to the coefficient combine.
+// Y[s'] += sum_s C_t[s'][s] Z[s].  The combine is software-pipelined: a term's products stay pending
+// (zp, cp) and are combined during the next term's matrix products (across runs and alphas; the caller
+// combines the last pending term), so the vector work overlaps the MFMAs.  The pending state starts as
+// zeros with zero coefficients.
 template <int P, int DP, typename T, int SPLIT>
-__device__ __attribute__((always_inline)) inline void lp_alpha(const LPDevPlan* __restrict__ plan, const LPChunk* __restrict__ C, int n_runs,
-                                const T* __restrict__ lds, const typename LPType<T>::vec8 (&vt)[DP * ((P + 15) / 16)][(P + 31) / 32],
+__device__ __attribute__((always_inline)) inline void lp_alpha(const LPMeta<DP>& meta, int n_runs, const T* __restrict__ lds,
+                                const typename LPType<T>::vec8 (&vt)[DP * ((P + 15) / 16)][(P + 31) / 32],
                                 const typename LPType<T>::vec8 (&vl)[DP * ((P + 15) / 16)][(P + 31) / 32],
-                                f32x4 (&y)[(P + 15) / 16][DP], int np, int lane) {
+                                f32x4 (&y)[(P + 15) / 16][DP], f32x4 (&zp)[(P + 15) / 16][DP], float (&cp)[DP * DP],
+                                int np, int lane) {
   typedef LPType<T> LT;
   typedef typename LT::vec8 V;
   constexpr int NT = (P + 15) / 16, QT = NT, KT = (P + 31) / 32, MT = DP * QT, SP = lp_stride(P);
   const int g = lane >> 4;
+  // B operands of the next term, loaded during the current term's products (the chunk's terms run in
+  // order across its runs; the last term reloads itself)
+  const int tlast = lp_lane_int(meta.rt0, n_runs) - 1;
+  V bop[KT][NT];
+  {
+    const T* Bs = lds + lp_lane_int(meta.tslot, lp_lane_int(meta.rt0, 0)) * SP;
+#pragma unroll
+    for (int t = 0; t < KT; ++t)
+#pragma unroll
+      for (int nq = 0; nq < NT; ++nq) bop[t][nq] = lds8<T, P>(Bs, 16 * nq + (lane & 15), t, g);
+  }
+  // A operands of the next run, likewise loaded during the current run's terms
+  V aop[KT];
+  {
+    const T* As = lds + lp_lane_int(meta.rslot, 0) * SP;
+#pragma unroll
+    for (int t = 0; t < KT; ++t) aop[t] = lds8<T, P>(As, 16 * np + (lane & 15), t, g);
+  }
   for (int r = 0; r < n_runs; ++r) {
-    const T* As = lds + C->run_slot[r] * SP;
+    const int t0 = lp_lane_int(meta.rt0, r), t1 = lp_lane_int(meta.rt0, r + 1);
     V whi[DP][KT], wlo[DP][KT];
     {
       f32x4 w[MT];
@@ -176,13 +229,17 @@ __device__ __attribute__((always_inline)) inline void lp_alpha(const LPDevPlan* 
 #pragma unroll
       for (int t = 0; t < KT; ++t) {
         // hi products of all MT tiles, then the lo products: dependent MFMAs are MT issues apart
-        const V bop = lds8<T, P>(As, 16 * np + (lane & 15), t, g);
 #pragma unroll
-        for (int m = 0; m < MT; ++m) w[m] = LT::mfma(vt[m][t], bop, w[m]);
+        for (int m = 0; m < MT; ++m) w[m] = LT::mfma(vt[m][t], aop[t], w[m]);
         if (SPLIT != LP_SPLIT_NONE) {
 #pragma unroll
-          for (int m = 0; m < MT; ++m) w[m] = LT::mfma(vl[m][t], bop, w[m]);
+          for (int m = 0; m < MT; ++m) w[m] = LT::mfma(vl[m][t], aop[t], w[m]);
         }
+      }
+      {
+        const T* As = lds + lp_lane_int(meta.rslot, r + 1 < n_runs ? r + 1 : r) * SP;
+#pragma unroll
+        for (int t = 0; t < KT; ++t) aop[t] = lds8<T, P>(As, 16 * np + (lane & 15), t, g);
       }
 #pragma unroll
       for (int s = 0; s < DP; ++s)
@@ -205,9 +262,10 @@ __device__ __attribute__((always_inline)) inline void lp_alpha(const LPDevPlan* 
             }
           }
     }
-    const int t1 = C->run_t0[r + 1];
-    for (int tt = C->run_t0[r]; tt < t1; ++tt) {
-      const T* Bs = lds + plan->term_slot[tt] * SP;
+    for (int tt = t0; tt < t1; ++tt) {
+      float cn[DP * DP];
+#pragma unroll
+      for (int k = 0; k < DP * DP; ++k) cn[k] = lp_lane_float(meta.coef[k], tt);
       f32x4 z[NT][DP];
 #pragma unroll
       for (int nq = 0; nq < NT; ++nq)
@@ -215,35 +273,32 @@ __device__ __attribute__((always_inline)) inline void lp_alpha(const LPDevPlan* 
         for (int s = 0; s < DP; ++s) z[nq][s] = (f32x4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int t = 0; t < KT; ++t) {
-        V bop[NT];
-#pragma unroll
-        for (int nq = 0; nq < NT; ++nq) bop[nq] = lds8<T, P>(Bs, 16 * nq + (lane & 15), t, g);
         // hi products of all NT*DP accumulators, then the lo products
 #pragma unroll
         for (int nq = 0; nq < NT; ++nq)
 #pragma unroll
-          for (int s = 0; s < DP; ++s) z[nq][s] = LT::mfma(whi[s][t], bop[nq], z[nq][s]);
+          for (int s = 0; s < DP; ++s) z[nq][s] = LT::mfma(whi[s][t], bop[t][nq], z[nq][s]);
         if (SPLIT == LP_SPLIT_BOTH) {
 #pragma unroll
           for (int nq = 0; nq < NT; ++nq)
 #pragma unroll
-            for (int s = 0; s < DP; ++s) z[nq][s] = LT::mfma(wlo[s][t], bop[nq], z[nq][s]);
+            for (int s = 0; s < DP; ++s) z[nq][s] = LT::mfma(wlo[s][t], bop[t][nq], z[nq][s]);
         }
       }
-      const float* cf = plan->coef[tt];
-      if (plan->term_diag[tt]) {
+      {
+        const T* Bs = lds + lp_lane_int(meta.tslot, tt < tlast ? tt + 1 : tlast) * SP;
 #pragma unroll
-        for (int nq = 0; nq < NT; ++nq)
+        for (int t = 0; t < KT; ++t)
 #pragma unroll
-          for (int s = 0; s < DP; ++s) y[nq][s] += cf[s * DP + s] * z[nq][s];
-      } else {
-#pragma unroll
-        for (int nq = 0; nq < NT; ++nq)
-#pragma unroll
-          for (int sp = 0; sp < DP; ++sp)
-#pragma unroll
-            for (int s = 0; s < DP; ++s) y[nq][sp] += cf[sp * DP + s] * z[nq][s];
+          for (int nq = 0; nq < NT; ++nq) bop[t][nq] = lds8<T, P>(Bs, 16 * nq + (lane & 15), t, g);
       }
+      lp_combine<NT, DP>(y, zp, cp);  // the previous term, independent of the products above
+#pragma unroll
+      for (int nq = 0; nq < NT; ++nq)
+#pragma unroll
+        for (int s = 0; s < DP; ++s) zp[nq][s] = z[nq][s];
+#pragma unroll
+      for (int k = 0; k < DP * DP; ++k) cp[k] = cn[k];
     }
   }
 }
@@ -332,18 +387,29 @@ __global__ void __launch_bounds__(64 * ((P + 15) / 16), 2) lp_kernel(const LPDev
 #pragma unroll
     for (int s = 0; s < DP; ++s) y[j][s] = (f32x4){0.f, 0.f, 0.f, 0.f};
 
+  const LPMeta<DP> meta = lp_meta<DP>(plan, C, lane);
+  f32x4 zp[NT][DP];
+  float cp[DP * DP];
+#pragma unroll
+  for (int j = 0; j < NT; ++j)
+#pragma unroll
+    for (int s = 0; s < DP; ++s) zp[j][s] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int k = 0; k < DP * DP; ++k) cp[k] = 0.f;
+
   for (int64_t al = a0; al < a1; al += 2) {
     // buffer 0 holds alpha al; everyone has finished reading buffer 1 (alpha al-1)
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     __syncthreads();
     if (al + 1 < a1) lp_stage<P, NT>(C, n_slots, al + 1, buf1, np, lane);
-    lp_alpha<P, DP, T, SPLIT>(plan, C, n_runs, reinterpret_cast<const T*>(buf0), vt, vl, y, np, lane);
+    lp_alpha<P, DP, T, SPLIT>(meta, n_runs, reinterpret_cast<const T*>(buf0), vt, vl, y, zp, cp, np, lane);
     if (al + 1 >= a1) break;
     __builtin_amdgcn_s_waitcnt(0x0F70);
     __syncthreads();
     if (al + 2 < a1) lp_stage<P, NT>(C, n_slots, al + 2, buf0, np, lane);
-    lp_alpha<P, DP, T, SPLIT>(plan, C, n_runs, reinterpret_cast<const T*>(buf1), vt, vl, y, np, lane);
+    lp_alpha<P, DP, T, SPLIT>(meta, n_runs, reinterpret_cast<const T*>(buf1), vt, vl, y, zp, cp, np, lane);
   }
+  lp_combine<NT, DP>(y, zp, cp);  // the last pending term
   // rows p' = 16 np + 4(lane/16) + r, columns q' = 16 nq + lane%16 of this workgroup's partial
   float* out = partial + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * NY;
 #pragma unroll
@@ -643,11 +709,7 @@ int lp_build_plan(LPDevPlan* dp, int64_t P, int64_t d, const void* const* A, con
       dp->term_slot[out_t] = slot_of(b_ptr);
       dp->term_a[out_t] = ta[k];
       dp->term_b[out_t] = tb[k];
-      dp->term_diag[out_t] = 1;
-      for (int e = 0; e < d * d; ++e) {
-        dp->coef[out_t][e] = coef[k * d * d + e];
-        if (e / d != e % d && coef[k * d * d + e] != 0.0f) dp->term_diag[out_t] = 0;
-      }
+      for (int e = 0; e < d * d; ++e) dp->coef[out_t][e] = coef[k * d * d + e];
       ++out_t;
     }
     if (run_open) {
