@@ -58,83 +58,97 @@ __device__ __forceinline__ void tri_sweep(double (&r)[CB], double s, F src) {
 // factors meet in each FMA).  A non-positive pivot of a real row records its 1-based index in bad and is
 // replaced by 1e-300.
 // Blocks of 64R x 64R on 4R waves: wave w holds columns 16w .. 16w + 15, lane i rows i + 64r (r < R).
-// At step K the wave owning column K forms the pivot and the multipliers, publishes them in LDS (two
-// buffers by step parity, so one barrier per step), and every wave with columns past K updates them.
-// Steps at or past the block's kb real rows only meet the identity padding and change nothing, so the
-// sweep ends there.
-constexpr int CW = 16;
-// Pivot of step K in its owner wave K / CW: the multipliers s_row into sb and c[.][K % CW] set.
-template <int K, int R>
-__device__ __forceinline__ void chol_pivot4(double (&c)[R][CW], double* sb, int i, int kb, int& bad) {
-  constexpr int k = K % CW;
-  double d = readlane_f64(c[K / 64][k], K % 64);
-  if (!(d > 0.0)) {
-    if (K < kb && bad == 0) bad = K + 1;
-    d = 1e-300;
-  }
-  // 1/sqrt(d): the hardware estimate and two Newton steps y += (y/2)(1 - d y^2) (each squares the
-  // relative error; within an ulp or so of the rounded quotient), sqrt(d) = d / sqrt(d); far shorter
-  // than a correctly rounded square root followed by a division on this step's critical path
-  double inv = __builtin_amdgcn_rsq(d);
+// The sweep goes in panels of PB = 4 columns.  The wave owning a panel factors it alone: per step the
+// pivot, the multipliers (published in LDS) and the step's update of the panel's later columns, with the
+// row multipliers it needs taken by readlane.  After one barrier every wave with columns past the panel
+// applies the panel's four updates, in step order, so each element sees the same FMAs in the same order
+// as in a column-by-column sweep.  The owner of the next panel first brings that panel's columns up to
+// date and factors it into the other LDS buffer (look-ahead), so the pivot chain overlaps the other
+// columns' updates.  Panels at or past the block's kb real rows only meet the identity padding and change
+// nothing, so the sweep ends there (steps past kb inside the last panel leave the real rows unchanged).
+constexpr int CW = 16, PB = 4;
+// Steps K .. KE - 1 of one panel in its owner wave (wave K / CW); multipliers of step K into sp[K % PB].
+template <int K, int KE, int R>
+__device__ __forceinline__ void chol_panel_steps(double (&c)[R][CW], double (*sp)[64 * R], int i, int kb, int& bad) {
+  if constexpr (K < KE) {
+    constexpr int k = K % CW;
+    double d = readlane_f64(c[K / 64][k], K % 64);
+    if (!(d > 0.0)) {
+      if (K < kb && bad == 0) bad = K + 1;
+      d = 1e-300;
+    }
+    // 1/sqrt(d): the hardware estimate and two Newton steps y += (y/2)(1 - d y^2) (each squares the
+    // relative error; within an ulp or so of the rounded quotient), sqrt(d) = d / sqrt(d); far shorter
+    // than a correctly rounded square root followed by a division on this step's critical path
+    double inv = __builtin_amdgcn_rsq(d);
 #pragma unroll
-  for (int it = 0; it < 2; ++it) inv = fma(0.5 * inv, fma(-d * inv, inv, 1.0), inv);
-  const double sd = d * inv;
+    for (int it = 0; it < 2; ++it) inv = fma(0.5 * inv, fma(-d * inv, inv, 1.0), inv);
+    const double sd = d * inv;
+    double sv[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int row = i + 64 * r;
-    const double sv = (row == K) ? 0.0 : c[r][k] * inv;
-    c[r][k] = (row == K) ? sd : sv;
-    sb[row] = sv;
+    for (int r = 0; r < R; ++r) {
+      const int row = i + 64 * r;
+      sv[r] = (row == K) ? 0.0 : c[r][k] * inv;
+      c[r][k] = (row == K) ? sd : sv[r];
+      sp[K % PB][row] = sv[r];
+    }
+    // this step on the panel's later columns j (row j's multiplier from its lane)
+#pragma unroll
+    for (int j = K + 1; j < KE; ++j) {
+      const double sj = readlane_f64(sv[j / 64], j % 64);
+#pragma unroll
+      for (int r = 0; r < R; ++r) c[r][j % CW] -= sv[r] * sj;
+    }
+    chol_panel_steps<K + 1, KE, R>(c, sp, i, kb, bad);
   }
 }
 
-// Step K (its multipliers already in sbuf[K & 1]): after the barrier the owner of column K + 1 applies
-// step K to that column first and forms the pivot of step K + 1 into the other buffer (look-ahead), so
-// the pivot chain overlaps the other columns' updates.
-template <int K, int R>
-__device__ __forceinline__ void chol_inv_steps4(double (&c)[R][CW], double (*sbuf)[64 * R], int w, int i, int kb, int& bad) {
-  if constexpr (K < 64 * R) {
-    if (K >= kb) return;
-    constexpr int o = K / CW, k = K % CW;
-    constexpr bool ahead = K + 1 < 64 * R;
-    constexpr int o1 = ahead ? (K + 1) / CW : 0, k1 = ahead ? (K + 1) % CW : 0;
+// Panel K0 .. K0 + PB - 1 (its multipliers already in sbuf[(K0 / PB) & 1]) applied past the panel.
+template <int K0, int R>
+__device__ __forceinline__ void chol_inv_panels(double (&c)[R][CW], double (*sbuf)[PB][64 * R], int w, int i, int kb, int& bad) {
+  if constexpr (K0 < 64 * R) {
+    if (K0 >= kb) return;
+    constexpr int o = K0 / CW, k0 = K0 % CW;
+    constexpr bool ahead = K0 + PB < 64 * R;
+    constexpr int K1 = ahead ? K0 + PB : 0, o1 = K1 / CW, k1 = K1 % CW;
     __syncthreads();
-    const double* sb = sbuf[K & 1];
+    double (*sp)[64 * R] = sbuf[(K0 / PB) & 1];
     if (w >= o) {
-      double si[R];
+      double si[PB][R];
 #pragma unroll
-      for (int r = 0; r < R; ++r) si[r] = sb[i + 64 * r];
+      for (int kk = 0; kk < PB; ++kk)
+#pragma unroll
+        for (int r = 0; r < R; ++r) si[kk][r] = sp[kk][i + 64 * r];
       if (ahead && w == o1) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) c[r][k1] -= si[r] * sb[K + 1];
-        if constexpr (ahead) chol_pivot4<K + 1, R>(c, sbuf[(K + 1) & 1], i, kb, bad);
-      }
-      if (w == o) {
-        // columns past K in the owner wave, less column K + 1 when the look-ahead took it
-        constexpr int j0 = (ahead && o1 == o) ? k + 2 : k + 1;
+        for (int jj = k1; jj < k1 + PB; ++jj)
 #pragma unroll
-        for (int jj = j0; jj < CW; ++jj) {
-          const double sj = sb[CW * o + jj];
+          for (int kk = 0; kk < PB; ++kk) {
+            const double sj = sp[kk][CW * o1 + jj];
 #pragma unroll
-          for (int r = 0; r < R; ++r) c[r][jj] -= si[r] * sj;
-        }
-      } else {
-        const double* sw = sb + CW * w;
-        const int j0 = (ahead && o1 != o && w == o1) ? 1 : 0;  // column 0 taken by the look-ahead
-#pragma unroll
-        for (int jj = 0; jj < CW; ++jj)
-          if (jj >= j0) {
-            const double sj = sw[jj];
-#pragma unroll
-            for (int r = 0; r < R; ++r) c[r][jj] -= si[r] * sj;
+            for (int r = 0; r < R; ++r) c[r][jj] -= si[kk][r] * sj;
           }
+        if constexpr (ahead) chol_panel_steps<K1, K1 + PB, R>(c, sbuf[(K1 / PB) & 1], i, kb, bad);
       }
+      // this wave's columns past the panel, less the next panel's when the look-ahead took them
+      const double* sw = &sp[0][0] + CW * w;
+      const int j0 = (w == o) ? k0 + PB : 0;
+#pragma unroll
+      for (int jj = 0; jj < CW; ++jj)
+        if (jj >= j0 && !(ahead && w == o1 && jj >= k1 && jj < k1 + PB)) {
+#pragma unroll
+          for (int kk = 0; kk < PB; ++kk) {
+            const double sj = sw[kk * 64 * R + jj];
+#pragma unroll
+            for (int r = 0; r < R; ++r) c[r][jj] -= si[kk][r] * sj;
+          }
+        }
     }
-    chol_inv_steps4<K + 1, R>(c, sbuf, w, i, kb, bad);
+    chol_inv_panels<K0 + PB, R>(c, sbuf, w, i, kb, bad);
   }
 }
 
-// Cholesky and inverse of one diagonal block of at most 64R x 64R (see chol_inv_steps4): A's block at
+// Cholesky and inverse of one diagonal block of at most 64R x 64R (see chol_inv_panels): A's block at
 // (k0, k0) (lower triangle read; rows >= kb padded with the identity) is replaced by L with its upper
 // triangle zeroed, X's block at (x0, x0) receives L^-1, info the first non-positive pivot (1-based,
 // global index).
@@ -144,7 +158,7 @@ __global__ void __launch_bounds__(256 * R) chol_leaf_kernel(int64_t lda, int64_t
                                                          int* __restrict__ info) {
   constexpr int BN = 64 * R, NW = BN / CW;
   __shared__ double Ls[BN][BN + 1];
-  __shared__ double sbuf[2][BN];
+  __shared__ double sbuf[2][PB][BN];
   __shared__ int badw[NW];
   const int i = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   double c[R][CW];
@@ -161,8 +175,8 @@ __global__ void __launch_bounds__(256 * R) chol_leaf_kernel(int64_t lda, int64_t
     }
   }
   int bad = 0;  // first failing pivot among this wave's columns
-  if (w == 0) chol_pivot4<0, R>(c, sbuf[0], i, kb, bad);
-  chol_inv_steps4<0, R>(c, sbuf, w, i, kb, bad);
+  if (w == 0) chol_panel_steps<0, PB, R>(c, sbuf[0], i, kb, bad);
+  chol_inv_panels<0, R>(c, sbuf, w, i, kb, bad);
   if (i == 0) badw[w] = bad;
 #pragma unroll
   for (int r = 0; r < R; ++r)
