@@ -23,6 +23,28 @@ def test_contract_with_matches_tensordot(gpu):
         np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
 
 
+def test_gemm_tile_shapes_match_numpy(gpu):
+    """cc_gemm_f64 on its tile shapes, 32 x 32 (small grids) and 64 x 64 (1296 x 1296 x 1100, the shape of
+    the normalization environment's product; 1300 x 1290 leaves ragged edges on both tile dimensions),
+    with K-contiguous and row-contiguous operands, alpha and beta."""
+    import torch
+    from carcassonne_amd import _lib
+    from carcassonne_amd.device import ptr, stream
+    rng = _rng(4)
+    for M, N, K, a_t, b_t in ((40, 30, 50, False, False), (700, 600, 96, True, False),
+                              (1296, 1296, 1100, True, False), (1300, 1290, 1030, False, True)):
+        a = rng.standard_normal((K, M) if a_t else (M, K))
+        b = rng.standard_normal((N, K) if b_t else (K, N))
+        c0 = rng.standard_normal((M, N))
+        A, B, C = (torch.from_numpy(x.copy()).cuda() for x in (a, b, c0))
+        sAm, sAk = (1, M) if a_t else (K, 1)
+        sBk, sBn = (1, K) if b_t else (N, 1)
+        _lib.call("cc_gemm_f64", stream(), 1, M, N, K, 0.5, ptr(A), 0, sAm, sAk, ptr(B), 0, sBk, sBn, 2.0,
+                  ptr(C), 0, N, 1)
+        want = 0.5 * ((a.T if a_t else a) @ (b.T if b_t else b)) + 2.0 * c0
+        np.testing.assert_allclose(C.cpu().numpy(), want, rtol=1e-12, atol=1e-11 * np.sqrt(K))
+
+
 def test_join_transpose_and_elementwise(gpu):
     from carcassonne_amd.data import HipData
     rng = _rng(2)
