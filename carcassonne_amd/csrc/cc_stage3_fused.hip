@@ -125,12 +125,16 @@ __global__ void __launch_bounds__(256) fused_f64_kernel(const FusedPlan plan, co
         f64x4 w[MT];
 #pragma unroll
         for (int m = 0; m < MT; ++m) w[m] = (f64x4){0.0, 0.0, 0.0, 0.0};
+        // the V^T operands are read from LDS at each use (the empty asm hides that the address is the
+        // same every time): keeping all of them in registers would leave too few for operand loads
+        int vo = lane;
+        __asm__ volatile("" : "+v"(vo));
 #pragma unroll
         for (int u = 0; u < KS; ++u) {
           double bop = Ap[(u * NT + np) * 64 + lane];
 #pragma unroll
           for (int m = 0; m < MT; ++m)
-            w[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(lds[(m * KS + u) * 64 + lane], bop, w[m], 0, 0, 0);
+            w[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(lds[(m * KS + u) * 64 + vo], bop, w[m], 0, 0, 0);
         }
         // stage B for every term of channel a, p' tile np
         for (int tt = t_begin; tt < t_end; ++tt) {
