@@ -91,3 +91,24 @@ def test_sampled_launch_timing_counts(gpu):
     assert kms[1] > 0.0
     with pytest.raises(Exception):
         _lib.call("cc_profile_begin_sampled", 0)
+
+
+@pytest.mark.parametrize("model", ["ising", "heisenberg"])
+def test_lowprec_plan_runs_and_chunks_at_d6(gpu, model):
+    """The FP16 kernel reads its term and run tables by lane and loads the next term's and run's operands
+    ahead (across runs and chunk boundaries): at D=6 (P=36), with the default grouping and with one
+    workgroup per alpha pair, it gives the FP64 result to the FP16 tolerance.  TFIM (7 terms) fits one
+    chunk; Heisenberg (14 terms) needs two, so the chunk-boundary path runs."""
+    from carcassonne_amd.precision import LowPrecisionOperator
+    H, N, st = _operators(6, 12, model, 9)
+    x = torch.from_numpy(st.ravel()).cuda()
+    y64 = torch.empty_like(x)
+    H.apply_flat(x, y64)
+    for groups in (None, H.n_alpha // 2):
+        lp = LowPrecisionOperator(H, "f16", groups=groups)
+        y = torch.empty_like(x)
+        lp.apply_flat(x, y)
+        err = float((y - y64).norm() / y64.norm())
+        print(model, "terms", len(H._A), "chunks", lp.chunks, "groups", lp.groups, "err", err)
+        assert lp.chunks == (1 if model == "ising" else 2), (model, lp.chunks)
+        assert err < 3e-3, (model, groups, err)
