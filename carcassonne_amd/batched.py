@@ -8,7 +8,7 @@ forked from the capture stream, so one `replay` launches every solve and the GPU
 systems' kernels.  An autotuner picks, per shape bucket, the low-precision kernel's workgroup count by
 timing candidates on the bucket's first system, and the fixed work (Lanczos steps, refinement passes)
 from one adaptive solve per bucket with a safety margin.  The plain FP64 solve of every system is the
-judge of the batched energies.
+reference for the batched energies.
 """
 import time
 
@@ -144,7 +144,7 @@ class BatchedSolve:
         return [float(rho.item()) for _, rho in self.outputs]
 
     def reference_energies(self):
-        """Plain FP64 solve of every system (the judge)."""
+        """Plain FP64 solve of every system (the reference energies)."""
         out = []
         for sysm in self.systems:
             from .solver import GroundStateSolver

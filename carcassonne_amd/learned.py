@@ -6,7 +6,7 @@ No reference counterpart: the reference always starts relaxOver from the current
 spectra of the four corners and four sides (identity channel), each divided by its largest value,
 plus the field h -- to the D^4 d center tensor.  It is trained on (environment, converged state)
 pairs that the plain FP64 solve produces, with the HIP forward / backward / Adam kernels of
-cc_mlp.hip.  The plain solve stays the judge: a proposal is only ever a start vector, the solve
+cc_mlp.hip.  The plain solve stays the arbiter: a proposal is only ever a start vector, the solve
 still runs to its own tolerance.
 
 Every numerical step runs in the HIP library; torch tensors are buffers only.

@@ -244,7 +244,7 @@ int cc_normalize_f64(void* stream, int64_t n, const double* x, double* y);
 
 /* ---- learned warm start: a 3-layer MLP proposing the eigensolver's start vector ----------------
  * (no reference counterpart: the reference always starts relaxOver from the current center,
- * system/_2d.py:491-492; the plain solve stays the judge of every proposal)
+ * system/_2d.py:491-492; the plain solve stays the arbiter of every proposal)
  * Network x (K0) -> relu(W1 x + b1) (H) -> relu(W2 . + b2) (H) -> W3 . + b3 (NO); parameters in one FP32
  * buffer [W1 (H x K0), b1, W2 (H x H), b2, W3 (NO x H), b3] (cc_mlp_param_count floats) with an FP16
  * mirror P16 of the same layout.  K0 % 16 == 0; H, NO multiples of 64; X16 is B x K0 FP16.

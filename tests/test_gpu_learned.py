@@ -1,6 +1,6 @@
 """The learned warm start on the GPU: features, forward pass and gradients of the HIP kernels against
 the NumPy oracle (oracle/mlp.py), one Adam step against its formula, and the gain of a trained network
-over random starts on held-out environments of a family (the plain solve judges every start)."""
+over random starts on held-out environments of a family (the plain solve checks every start)."""
 import numpy as np
 import pytest
 import torch
