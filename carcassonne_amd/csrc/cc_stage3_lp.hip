@@ -33,6 +33,9 @@ constexpr int LP_MAX_CHUNKS = 16, LP_CHUNK_SLOTS = 32;
 // Which operands carry a hi + lo pair: none (every operand rounded once), the vector and the stage-B
 // intermediate (only the slabs rounded), or the vector only (the stage-B intermediate rounded once).
 enum { LP_SPLIT_NONE = 0, LP_SPLIT_BOTH = 1, LP_SPLIT_VECTOR = 2 };
+// Coefficient pattern of a term: a full C_t, a diagonal one, or (d = 2) an anti-diagonal one, which is
+// diagonal once the stage-B products take the other half of W (Z'[s] = W[1 - s] B, Y[s] += C[s][1 - s] Z'[s]).
+enum { LP_FULL = 0, LP_DIAG = 1, LP_ANTI = 2 };
 typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) void lds_void;
@@ -57,6 +60,7 @@ struct LPChunk {
   int n_slots, n_runs;
   int run_slot[LP_MAX_TERMS];      // LDS slot of run r's A slab
   int run_t0[LP_MAX_TERMS + 1];    // run r covers chunk-ordered terms [run_t0[r], run_t0[r+1])
+  int run_tm[LP_MAX_TERMS];        // ... of which [run_t0[r], run_tm[r]) are anti-diagonal (listed first)
   const void* slot_ptr[LP_CHUNK_SLOTS];
 };
 struct LPDevPlan {
@@ -157,7 +161,9 @@ template <int DP>
 struct LPMeta {
   int tslot;            // term l's B slot
   float coef[DP * DP];  // term l's coefficient matrix C_t[s'][s] at s' DP + s (diagonal terms too)
-  int rslot, rt0;       // run l's A slot and first term (rt0 of lane n_runs: the chunk's term count)
+  float cd[DP];         // its diagonal form: C[s][s], or C[s][1 - s] for an anti-diagonal term
+  int rslot, rt0, rtm;  // run l's A slot, first term (rt0 of lane n_runs: the chunk's term count) and
+                        // first term past its anti-diagonal ones
 };
 template <int DP>
 __device__ inline LPMeta<DP> lp_meta(const LPDevPlan* __restrict__ plan, const LPChunk* __restrict__ C, int lane) {
@@ -166,12 +172,25 @@ __device__ inline LPMeta<DP> lp_meta(const LPDevPlan* __restrict__ plan, const L
   m.tslot = plan->term_slot[t];
 #pragma unroll
   for (int k = 0; k < DP * DP; ++k) m.coef[k] = plan->coef[t][k];
+  const bool anti = DP == 2 && plan->coef[t][0] == 0.f && plan->coef[t][DP * DP - 1] == 0.f;
+#pragma unroll
+  for (int sp = 0; sp < DP; ++sp) m.cd[sp] = plan->coef[t][sp * DP + (anti ? DP - 1 - sp : sp)];
   m.rslot = C->run_slot[t];
+  m.rtm = C->run_tm[t];
   m.rt0 = C->run_t0[lane <= LP_MAX_TERMS ? lane : LP_MAX_TERMS];
   return m;
 }
 __device__ inline int lp_lane_int(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 __device__ inline float lp_lane_float(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+
+// Y[s] += C[s] Z[s] for a term in diagonal form
+template <int NT, int DP>
+__device__ __forceinline__ void lp_combine_diag(f32x4 (&y)[NT][DP], const f32x4 (&z)[NT][DP], const float (&c)[DP]) {
+#pragma unroll
+  for (int nq = 0; nq < NT; ++nq)
+#pragma unroll
+    for (int s = 0; s < DP; ++s) y[nq][s] += c[s] * z[nq][s];
+}
 
 // Y[s'] += sum_s C[s'][s] Z[s] for one term's products
 template <int NT, int DP>
@@ -184,6 +203,52 @@ __device__ __forceinline__ void lp_combine(f32x4 (&y)[NT][DP], const f32x4 (&z)[
       for (int s = 0; s < DP; ++s) y[nq][sp] += c[sp * DP + s] * z[nq][s];
 }
 
+// One term of lp_alpha's diagonal-form loop: its products (with the other half of W when SWAP, an
+// anti-diagonal term), the next term's operands into bop (Bn), and the pending previous term's
+// diagonal-form combine, in one block so that the scheduler interleaves the combine with the MFMAs;
+// the term then becomes the pending one.
+template <int P, int DP, typename T, int SPLIT, bool SWAP>
+__device__ __forceinline__ void lp_term_d(f32x4 (&y)[(P + 15) / 16][DP], f32x4 (&zp)[(P + 15) / 16][DP], float (&cpd)[DP],
+                                          const typename LPType<T>::vec8 (&whi)[DP][(P + 31) / 32],
+                                          const typename LPType<T>::vec8 (&wlo)[DP][(P + 31) / 32],
+                                          typename LPType<T>::vec8 (&bop)[(P + 31) / 32][(P + 15) / 16], const T* Bn,
+                                          const LPMeta<DP>& meta, int tt, int lane, int g) {
+  typedef LPType<T> LT;
+  constexpr int NT = (P + 15) / 16, KT = (P + 31) / 32;
+  float cdn[DP];
+#pragma unroll
+  for (int k = 0; k < DP; ++k) cdn[k] = lp_lane_float(meta.cd[k], tt);
+  f32x4 z[NT][DP];
+#pragma unroll
+  for (int nq = 0; nq < NT; ++nq)
+#pragma unroll
+    for (int s = 0; s < DP; ++s) z[nq][s] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int t = 0; t < KT; ++t) {
+#pragma unroll
+    for (int nq = 0; nq < NT; ++nq)
+#pragma unroll
+      for (int s = 0; s < DP; ++s) z[nq][s] = LT::mfma(whi[SWAP ? DP - 1 - s : s][t], bop[t][nq], z[nq][s]);
+    if (SPLIT == LP_SPLIT_BOTH) {
+#pragma unroll
+      for (int nq = 0; nq < NT; ++nq)
+#pragma unroll
+        for (int s = 0; s < DP; ++s) z[nq][s] = LT::mfma(wlo[SWAP ? DP - 1 - s : s][t], bop[t][nq], z[nq][s]);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < KT; ++t)
+#pragma unroll
+    for (int nq = 0; nq < NT; ++nq) bop[t][nq] = lds8<T, P>(Bn, 16 * nq + (lane & 15), t, g);
+  lp_combine_diag<NT, DP>(y, zp, cpd);
+#pragma unroll
+  for (int nq = 0; nq < NT; ++nq)
+#pragma unroll
+    for (int s = 0; s < DP; ++s) zp[nq][s] = z[nq][s];
+#pragma unroll
+  for (int k = 0; k < DP; ++k) cpd[k] = cdn[k];
+}
+
 // One alpha of one chunk for wave np (p' tile np).  Stage A: W tiles m = s*QT + mq over K tiles of p,
 // converted once per A run to hi + lo stage-B operands (element e < 4 from W tile s*QT + 2t, e >= 4 from
 // tile s*QT + 2t + 1, zero past QT).  Per term: Z[s] = W[s] B over K tiles of q, then
@@ -191,12 +256,12 @@ __device__ __forceinline__ void lp_combine(f32x4 (&y)[NT][DP], const f32x4 (&z)[
 // (zp, cp) and are combined during the next term's matrix products (across runs and alphas; the caller
 // combines the last pending term), so the vector work overlaps the MFMAs.  The pending state starts as
 // zeros with zero coefficients.
-template <int P, int DP, typename T, int SPLIT>
+template <int P, int DP, typename T, int SPLIT, bool DIAGF>
 __device__ __attribute__((always_inline)) inline void lp_alpha(const LPMeta<DP>& meta, int n_runs, const T* __restrict__ lds,
                                 const typename LPType<T>::vec8 (&vt)[DP * ((P + 15) / 16)][(P + 31) / 32],
                                 const typename LPType<T>::vec8 (&vl)[DP * ((P + 15) / 16)][(P + 31) / 32],
                                 f32x4 (&y)[(P + 15) / 16][DP], f32x4 (&zp)[(P + 15) / 16][DP], float (&cp)[DP * DP],
-                                int np, int lane) {
+                                float (&cpd)[DP], int np, int lane) {
   typedef LPType<T> LT;
   typedef typename LT::vec8 V;
   constexpr int NT = (P + 15) / 16, QT = NT, KT = (P + 31) / 32, MT = DP * QT, SP = lp_stride(P);
@@ -262,48 +327,60 @@ __device__ __attribute__((always_inline)) inline void lp_alpha(const LPMeta<DP>&
             }
           }
     }
-    for (int tt = t0; tt < t1; ++tt) {
-      float cn[DP * DP];
+    if constexpr (DIAGF) {
+      // every term diagonal or anti-diagonal: the run's anti-diagonal terms (listed first) take the other
+      // half of W, and every combine is in diagonal form
+      const int tm = lp_lane_int(meta.rtm, r);
+      for (int tt = t0; tt < tm; ++tt)
+        lp_term_d<P, DP, T, SPLIT, true>(y, zp, cpd, whi, wlo, bop,
+                                         lds + lp_lane_int(meta.tslot, tt < tlast ? tt + 1 : tlast) * SP, meta, tt, lane, g);
+      for (int tt = tm; tt < t1; ++tt)
+        lp_term_d<P, DP, T, SPLIT, false>(y, zp, cpd, whi, wlo, bop,
+                                          lds + lp_lane_int(meta.tslot, tt < tlast ? tt + 1 : tlast) * SP, meta, tt, lane, g);
+    } else {
+      for (int tt = t0; tt < t1; ++tt) {
+        float cn[DP * DP];
 #pragma unroll
-      for (int k = 0; k < DP * DP; ++k) cn[k] = lp_lane_float(meta.coef[k], tt);
-      f32x4 z[NT][DP];
-#pragma unroll
-      for (int nq = 0; nq < NT; ++nq)
-#pragma unroll
-        for (int s = 0; s < DP; ++s) z[nq][s] = (f32x4){0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int t = 0; t < KT; ++t) {
-        // hi products of all NT*DP accumulators, then the lo products
+        for (int k = 0; k < DP * DP; ++k) cn[k] = lp_lane_float(meta.coef[k], tt);
+        f32x4 z[NT][DP];
 #pragma unroll
         for (int nq = 0; nq < NT; ++nq)
 #pragma unroll
-          for (int s = 0; s < DP; ++s) z[nq][s] = LT::mfma(whi[s][t], bop[t][nq], z[nq][s]);
-        if (SPLIT == LP_SPLIT_BOTH) {
+          for (int s = 0; s < DP; ++s) z[nq][s] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < KT; ++t) {
+          // hi products of all NT*DP accumulators, then the lo products
 #pragma unroll
           for (int nq = 0; nq < NT; ++nq)
 #pragma unroll
-            for (int s = 0; s < DP; ++s) z[nq][s] = LT::mfma(wlo[s][t], bop[t][nq], z[nq][s]);
+            for (int s = 0; s < DP; ++s) z[nq][s] = LT::mfma(whi[s][t], bop[t][nq], z[nq][s]);
+          if (SPLIT == LP_SPLIT_BOTH) {
+#pragma unroll
+            for (int nq = 0; nq < NT; ++nq)
+#pragma unroll
+              for (int s = 0; s < DP; ++s) z[nq][s] = LT::mfma(wlo[s][t], bop[t][nq], z[nq][s]);
+          }
         }
+        {
+          const T* Bs = lds + lp_lane_int(meta.tslot, tt < tlast ? tt + 1 : tlast) * SP;
+#pragma unroll
+          for (int t = 0; t < KT; ++t)
+#pragma unroll
+            for (int nq = 0; nq < NT; ++nq) bop[t][nq] = lds8<T, P>(Bs, 16 * nq + (lane & 15), t, g);
+        }
+        lp_combine<NT, DP>(y, zp, cp);  // the previous term, independent of the products above
+#pragma unroll
+        for (int nq = 0; nq < NT; ++nq)
+#pragma unroll
+          for (int s = 0; s < DP; ++s) zp[nq][s] = z[nq][s];
+#pragma unroll
+        for (int k = 0; k < DP * DP; ++k) cp[k] = cn[k];
       }
-      {
-        const T* Bs = lds + lp_lane_int(meta.tslot, tt < tlast ? tt + 1 : tlast) * SP;
-#pragma unroll
-        for (int t = 0; t < KT; ++t)
-#pragma unroll
-          for (int nq = 0; nq < NT; ++nq) bop[t][nq] = lds8<T, P>(Bs, 16 * nq + (lane & 15), t, g);
-      }
-      lp_combine<NT, DP>(y, zp, cp);  // the previous term, independent of the products above
-#pragma unroll
-      for (int nq = 0; nq < NT; ++nq)
-#pragma unroll
-        for (int s = 0; s < DP; ++s) zp[nq][s] = z[nq][s];
-#pragma unroll
-      for (int k = 0; k < DP * DP; ++k) cp[k] = cn[k];
     }
   }
 }
 
-template <int P, int DP, typename T, int SPLIT>
+template <int P, int DP, typename T, int SPLIT, bool DIAGF>
 __global__ void __launch_bounds__(64 * ((P + 15) / 16), 2) lp_kernel(const LPDevPlan* __restrict__ plan,
                                                                      const double* __restrict__ v,
                                                                      double* __restrict__ scale,
@@ -389,7 +466,9 @@ __global__ void __launch_bounds__(64 * ((P + 15) / 16), 2) lp_kernel(const LPDev
 
   const LPMeta<DP> meta = lp_meta<DP>(plan, C, lane);
   f32x4 zp[NT][DP];
-  float cp[DP * DP];
+  float cp[DP * DP], cpd[DP];
+#pragma unroll
+  for (int k = 0; k < DP; ++k) cpd[k] = 0.f;
 #pragma unroll
   for (int j = 0; j < NT; ++j)
 #pragma unroll
@@ -402,14 +481,15 @@ __global__ void __launch_bounds__(64 * ((P + 15) / 16), 2) lp_kernel(const LPDev
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     __syncthreads();
     if (al + 1 < a1) lp_stage<P, NT>(C, n_slots, al + 1, buf1, np, lane);
-    lp_alpha<P, DP, T, SPLIT>(meta, n_runs, reinterpret_cast<const T*>(buf0), vt, vl, y, zp, cp, np, lane);
+    lp_alpha<P, DP, T, SPLIT, DIAGF>(meta, n_runs, reinterpret_cast<const T*>(buf0), vt, vl, y, zp, cp, cpd, np, lane);
     if (al + 1 >= a1) break;
     __builtin_amdgcn_s_waitcnt(0x0F70);
     __syncthreads();
     if (al + 2 < a1) lp_stage<P, NT>(C, n_slots, al + 2, buf0, np, lane);
-    lp_alpha<P, DP, T, SPLIT>(meta, n_runs, reinterpret_cast<const T*>(buf1), vt, vl, y, zp, cp, np, lane);
+    lp_alpha<P, DP, T, SPLIT, DIAGF>(meta, n_runs, reinterpret_cast<const T*>(buf1), vt, vl, y, zp, cp, cpd, np, lane);
   }
-  lp_combine<NT, DP>(y, zp, cp);  // the last pending term
+  if constexpr (DIAGF) lp_combine_diag<NT, DP>(y, zp, cpd);  // the last pending term
+  else lp_combine<NT, DP>(y, zp, cp);
   // rows p' = 16 np + 4(lane/16) + r, columns q' = 16 nq + lane%16 of this workgroup's partial
   float* out = partial + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * NY;
 #pragma unroll
@@ -602,6 +682,7 @@ inline LPLayout lp_layout(int64_t P, int64_t d, int64_t rows) {
 struct LPHandle {
   int64_t P, d;
   int dtype, groups, chunks, split;
+  int diagf;           // 1 when every term's coefficient matrix is diagonal or anti-diagonal (d = 2)
   float* work;         // lp_layout(P, d, groups * LP_MAX_CHUNKS); the device plan sits at its start
   LPDevPlan* host;     // the host plan, kept until destroy (source of the stream-ordered upload)
   hipEvent_t uploaded; // recorded after the upload; destroy waits for it before freeing `host`
@@ -620,9 +701,15 @@ int launch_lp(hipStream_t st, const LPHandle* h, const double* v, double* y) {
   hipEvent_t e0 = nullptr;
   int rc = cc_prof_start(st, CC_PROF_LP, &e0);
   if (rc != CC_OK) return rc;
-  if (h->split == LP_SPLIT_BOTH) lp_kernel<P, DP, T, LP_SPLIT_BOTH><<<grid, 64 * NT, 0, st>>>(plan, v, scale, partial);
-  else if (h->split == LP_SPLIT_VECTOR) lp_kernel<P, DP, T, LP_SPLIT_VECTOR><<<grid, 64 * NT, 0, st>>>(plan, v, scale, partial);
-  else lp_kernel<P, DP, T, LP_SPLIT_NONE><<<grid, 64 * NT, 0, st>>>(plan, v, scale, partial);
+  if (h->diagf) {
+    if (h->split == LP_SPLIT_BOTH) lp_kernel<P, DP, T, LP_SPLIT_BOTH, true><<<grid, 64 * NT, 0, st>>>(plan, v, scale, partial);
+    else if (h->split == LP_SPLIT_VECTOR) lp_kernel<P, DP, T, LP_SPLIT_VECTOR, true><<<grid, 64 * NT, 0, st>>>(plan, v, scale, partial);
+    else lp_kernel<P, DP, T, LP_SPLIT_NONE, true><<<grid, 64 * NT, 0, st>>>(plan, v, scale, partial);
+  } else {
+    if (h->split == LP_SPLIT_BOTH) lp_kernel<P, DP, T, LP_SPLIT_BOTH, false><<<grid, 64 * NT, 0, st>>>(plan, v, scale, partial);
+    else if (h->split == LP_SPLIT_VECTOR) lp_kernel<P, DP, T, LP_SPLIT_VECTOR, false><<<grid, 64 * NT, 0, st>>>(plan, v, scale, partial);
+    else lp_kernel<P, DP, T, LP_SPLIT_NONE, false><<<grid, 64 * NT, 0, st>>>(plan, v, scale, partial);
+  }
   CC_LAUNCH_CHECK();
   rc = cc_prof_stop(st, CC_PROF_LP, e0);
   if (rc != CC_OK) return rc;
@@ -653,6 +740,16 @@ __global__ void lp_scale_coef_kernel(LPDevPlan* plan, int n_terms, int dd, int n
   }
 }
 
+// Pattern of a d x d coefficient matrix (row-major): diagonal, anti-diagonal (d = 2, zero diagonal), or full.
+int lp_term_kind(const float* c, int d) {
+  bool diag = true, zero_diag = true;
+  for (int e = 0; e < d * d; ++e) {
+    if (c[e] != 0.0f && e / d != e % d) diag = false;
+    if (c[e] != 0.0f && e / d == e % d) zero_diag = false;
+  }
+  return diag ? LP_DIAG : (d == 2 && zero_diag) ? LP_ANTI : LP_FULL;
+}
+
 int lp_build_plan(LPDevPlan* dp, int64_t P, int64_t d, const void* const* A, const void* const* B, int n_terms,
                   const int* ta, const int* tb, const float* coef, int64_t n_alpha) {
   const int cap = lp_slots((int)P);
@@ -680,9 +777,14 @@ int lp_build_plan(LPDevPlan* dp, int64_t P, int64_t d, const void* const* A, con
     // terms of this A channel
     int t_end = t;
     while (t_end < n_terms && ta[t_end] == ta[t]) ++t_end;
+    // anti-diagonal terms first (each run lists them before the others; run_tm marks the split)
+    std::vector<int> order;
+    for (int pass = 0; pass < 2; ++pass)
+      for (int k = t; k < t_end; ++k)
+        if ((lp_term_kind(coef + k * d * d, (int)d) == LP_ANTI) == (pass == 0)) order.push_back(k);
     // place as many of them as fit; open a new chunk when even the first does not fit
     bool run_open = false;
-    for (int k = t; k < t_end; ++k) {
+    for (int k : order) {
       const void* b_ptr = B[tb[k]];
       const int need = (slot_of(a_ptr) < 0 ? 1 : 0) + (slot_of(b_ptr) < 0 ? 1 : 0);
       if ((int)slots.size() + need > cap) {
@@ -702,6 +804,7 @@ int lp_build_plan(LPDevPlan* dp, int64_t P, int64_t d, const void* const* A, con
         LPChunk& c = dp->chunk[chunk];
         c.run_slot[c.n_runs] = slot_of(a_ptr);
         c.run_t0[c.n_runs] = out_t;
+        c.run_tm[c.n_runs] = out_t;
         run_open = true;
       }
       if (slot_of(b_ptr) < 0) slots.push_back(b_ptr);
@@ -710,6 +813,7 @@ int lp_build_plan(LPDevPlan* dp, int64_t P, int64_t d, const void* const* A, con
       dp->term_a[out_t] = ta[k];
       dp->term_b[out_t] = tb[k];
       for (int e = 0; e < d * d; ++e) dp->coef[out_t][e] = coef[k * d * d + e];
+      if (lp_term_kind(coef + k * d * d, (int)d) == LP_ANTI) dp->chunk[chunk].run_tm[dp->chunk[chunk].n_runs] = out_t + 1;
       ++out_t;
     }
     if (run_open) {
@@ -835,6 +939,9 @@ extern "C" int cc_stage3_lp_plan_create(void** out, int dtype, int n_a, const vo
   h->groups = groups;
   h->chunks = dp->n_chunks;
   h->split = split;
+  h->diagf = d == 2;
+  for (int t = 0; t < n_terms; ++t)
+    if (lp_term_kind(dp->coef[t], (int)d) == LP_FULL) h->diagf = 0;
   h->work = work;
   h->host = dp;
   h->uploaded = ev;
