@@ -105,13 +105,19 @@ class BatchedSolve:
         self.autotune()
         self.build()
 
-    def tune_work(self, margin_iters=8, margin_outer=2):
-        """Fixed work per shape bucket from one adaptive solve of the bucket's first system."""
+    def tune_work(self, margin_iters=8, margin_outer=2, max_outer=None):
+        """Fixed work per shape bucket: the most low-precision Lanczos steps and FP64 passes that an
+        adaptive solve of any system of the bucket needed, plus margins.  BF16's correction equation
+        gains less per FP64 pass than FP16's, so its adaptive solves may take up to 24 passes."""
+        if max_outer is None:
+            max_outer = 24 if self.dtype == "bf16" else 8
         for key in self.tuning:
-            sysm = next(s for s in self.systems if s["key"] == key)
-            _, _, info = sysm["solver"].solve(sysm["x0"])
-            self.tuning[key]["lp_iters"] = int(info["lowprec_iterations"] * 1.25) + margin_iters
-            self.tuning[key]["outer"] = int(info["fp64_passes"]) + margin_outer
+            lp, passes = 0, 0
+            for sysm in (s for s in self.systems if s["key"] == key):
+                _, _, info = sysm["solver"].solve(sysm["x0"], max_outer=max_outer)
+                lp, passes = max(lp, int(info["lowprec_iterations"])), max(passes, int(info["fp64_passes"]))
+            self.tuning[key]["lp_iters"] = int(lp * 1.25) + margin_iters
+            self.tuning[key]["outer"] = passes + margin_outer
         synchronize()
 
     def _solve_all(self, streams):
@@ -141,7 +147,35 @@ class BatchedSolve:
 
     def energies(self):
         synchronize()
-        return [float(rho.item()) for _, rho in self.outputs]
+        return [float(rho.item()) for _, rho, _ in self.outputs]
+
+    def convergence(self, tol_change=1e-11, tol_residual=1e-7):
+        """Per system, from the graph's in-device record (read back after a replay): whether the last FP64
+        pass moved the energy by at most tol_change relative and the final relative residual
+        |H x - rho N x| / (|rho| |N x|) is at most tol_residual; returns (ok, change, residual) tuples."""
+        synchronize()
+        out = []
+        for _, _, rec in self.outputs:
+            h = rec.tolist()
+            rho, rr, nn = h[-3], h[-2], h[-1]
+            delta = abs(h[-3] - h[-4]) / abs(rho) if len(h) >= 4 else float("inf")
+            resid = float(np.sqrt(max(rr, 0.0) / max(nn, 1e-300))) / max(abs(rho), 1e-300)
+            out.append((bool(delta <= tol_change and resid <= tol_residual), delta, resid))
+        return out
+
+    def resolve_unconverged(self, conv=None):
+        """The guard's fallback: every system the record flags is solved again eagerly by the adaptive
+        mixed solve with the FP16 operator (its energy replaces the graph's).  Returns {index: energy}."""
+        conv = self.convergence() if conv is None else conv
+        out = {}
+        for i, (ok, _, _) in enumerate(conv):
+            if ok:
+                continue
+            sysm = self.systems[i]
+            solver = MixedSolver(sysm["H"].operator, sysm["N"].operator, dtype="f16", factor=sysm["solver"].factor)
+            _, rho, _ = solver.solve(sysm["x0"])
+            out[i] = rho
+        return out
 
     def reference_energies(self):
         """Plain FP64 solve of every system (the reference energies)."""

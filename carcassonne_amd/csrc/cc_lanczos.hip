@@ -469,7 +469,9 @@ __global__ void __launch_bounds__(1024) cg_step_kernel(int64_t n, const double* 
     ap[i] = a;
     pap += p[i] * a;
   }
-  const double alpha = rr[0] / cg_block_sum(pap, red);
+  const double pap_sum = cg_block_sum(pap, red);
+  // a converged (zero) residual or search direction ends the iteration with y unchanged, not a NaN
+  const double alpha = (pap_sum != 0.0 && rr[0] != 0.0) ? rr[0] / pap_sum : 0.0;
   double rn = 0.0;
   for (int64_t i = threadIdx.x; i < n; i += 1024) {
     y[i] += alpha * p[i];
@@ -478,7 +480,7 @@ __global__ void __launch_bounds__(1024) cg_step_kernel(int64_t n, const double* 
     rn += r * r;
   }
   const double rr_new = cg_block_sum(rn, red);
-  const double beta = rr_new / rr[0];
+  const double beta = rr[0] != 0.0 ? rr_new / rr[0] : 0.0;
   for (int64_t i = threadIdx.x; i < n; i += 1024) p[i] = res[i] + beta * p[i];
   __syncthreads();
   if (threadIdx.x == 0) rr[0] = rr_new;
@@ -516,7 +518,9 @@ __global__ void __launch_bounds__(1024) cg_step_reg_kernel(int64_t n, const doub
     a[k] -= c * uv[k];
     if (threadIdx.x + 1024 * k < n) pap += pv[k] * a[k];
   }
-  const double alpha = rr0 / cg_block_sum(pap, red);
+  const double pap_sum = cg_block_sum(pap, red);
+  // a converged (zero) residual or search direction ends the iteration with y unchanged, not a NaN
+  const double alpha = (pap_sum != 0.0 && rr0 != 0.0) ? rr0 / pap_sum : 0.0;
   double rn = 0.0;
 #pragma unroll
   for (int k = 0; k < R; ++k) {
@@ -525,7 +529,7 @@ __global__ void __launch_bounds__(1024) cg_step_reg_kernel(int64_t n, const doub
     if (threadIdx.x + 1024 * k < n) rn += rv[k] * rv[k];
   }
   const double rr_new = cg_block_sum(rn, red);
-  const double beta = rr_new / rr0;
+  const double beta = rr0 != 0.0 ? rr_new / rr0 : 0.0;
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     const int64_t i = threadIdx.x + 1024 * k;

@@ -955,6 +955,12 @@ extern "C" int cc_stage3_lp_plan_chunks(const void* plan, int* out) {
   return CC_OK;
 }
 
+extern "C" int cc_stage3_lp_plan_diagonal_form(const void* plan, int* out) {
+  if (!plan) { cc_set_last_error("null plan"); return CC_ERR_ARG; }
+  *out = static_cast<const LPHandle*>(plan)->diagf;
+  return CC_OK;
+}
+
 extern "C" int cc_stage3_lp_plan_apply(void* stream, const void* plan, const double* x, double* y) {
   if (!plan) { cc_set_last_error("null plan"); return CC_ERR_ARG; }
   return cc_internal_lp_apply(cc_stream(stream), plan, x, y);

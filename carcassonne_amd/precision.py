@@ -87,6 +87,9 @@ class LowPrecisionOperator:
         chunks = ctypes.c_int(0)
         _lib.call("cc_stage3_lp_plan_chunks", h, ctypes.byref(chunks))
         self.chunks = chunks.value
+        diag = ctypes.c_int(0)
+        _lib.call("cc_stage3_lp_plan_diagonal_form", h, ctypes.byref(diag))
+        self.diagonal_form = bool(diag.value)
         self.bytes_per_matvec = len(self._slabs) * self.n_alpha * self.P * self.P * 2 + 2 * self.n * 8
 
     def apply_flat(self, x, y):
@@ -154,6 +157,9 @@ class MixedSolver:
                 _lib.call("cc_basis_dots_f64", stream(), k, n, ptr(U), ptr(NU[j]), ptr(ns[:, j]), k)
             _lib.call("cc_small_geneig_f64", stream(), k, ptr(hs), ptr(ns), ptr(small[0:1]), ptr(small[1:1 + k]))
             rho = float(small[0].item())
+            if not np.isfinite(rho):
+                raise FloatingPointError("mixed-precision solve: the FP64 Rayleigh-Ritz value is not finite "
+                                         "(history {})".format(history))
             history.append(rho)
             coeffs = small[1:1 + k].contiguous()
             _lib.call("cc_basis_combine_f64", stream(), k, n, ptr(U), ptr(coeffs), ptr(xr))
@@ -183,14 +189,17 @@ class MixedSolver:
                 _lib.call("cc_axpby_f64", stream(), n, -1.0, ptr(tmp), 1.0, ptr(t))
             _lib.call("cc_normalize_f64", stream(), n, ptr(t), ptr(U[k]))
         f.check()
-        return xr, rho, dict(lowprec_iterations=lp_iters, outer_steps=k, fp64_passes=fp64_passes,
+        converged = len(history) >= 2 and abs(history[-1] - history[-2]) <= max(tol_energy, 1e-9) * abs(rho)
+        return xr, rho, dict(lowprec_iterations=lp_iters, outer_steps=k, fp64_passes=fp64_passes, converged=converged,
                              lowprec_ritz=float(res[0]), lowprec_residual=float(res[1]), history=history)
 
     def solve_fixed(self, x0, lp_iters=60, outer=3, inner_iters=20):
         """The same solve with a fixed amount of work and no host reads or synchronisation (so it can be
         captured in a HIP graph): lp_iters low-precision Lanczos steps, then `outer` FP64 Rayleigh-Ritz
-        passes with outer - 1 corrections between them.  Returns (x, rho) as device tensors; rho[0] is
-        the FP64 Rayleigh-Ritz value of x."""
+        passes with outer - 1 corrections between them.  Returns (x, rho, record) as device tensors;
+        rho[0] is the FP64 Rayleigh-Ritz value of x, record[k] (k < outer) that of pass k, and
+        record[outer], record[outer + 1] are |H x - rho N x|^2 and |N x|^2 of the final vector (the
+        convergence record that lets a caller check, after a replay, that the fixed work converged)."""
         from .data import HipData
         if isinstance(x0, HipData):
             x0 = x0._t.reshape(-1)
@@ -206,6 +215,7 @@ class MixedSolver:
         U, HU, NU = empty((kmax, n)), empty((kmax, n)), empty((kmax, n))
         small = empty((kmax + 1,))
         xr, r, t, tmp = empty((n,)), empty((n,)), empty((n,)), empty((n,))
+        hist = empty((outer + 2,))
         U[0].copy_(x)
         for k in range(1, outer + 1):
             _lib.call("cc_stage3_plan_apply", stream(), self._hplan, ptr(U[k - 1]), ptr(HU[k - 1]))
@@ -217,11 +227,14 @@ class MixedSolver:
             _lib.call("cc_small_geneig_f64", stream(), k, ptr(hs), ptr(ns), ptr(small[0:1]), ptr(small[1:1 + k]))
             coeffs = small[1:1 + k]
             _lib.call("cc_basis_combine_f64", stream(), k, n, ptr(U), ptr(coeffs), ptr(xr))
-            if k == outer:
-                break
+            _lib.call("cc_axpby_f64", stream(), 1, 1.0, ptr(small[0:1]), 0.0, ptr(hist[k - 1:k]))
             _lib.call("cc_basis_combine_f64", stream(), k, n, ptr(HU), ptr(coeffs), ptr(r))
             _lib.call("cc_basis_combine_f64", stream(), k, n, ptr(NU), ptr(coeffs), ptr(tmp))
             _lib.call("cc_axpby_dev_f64", stream(), n, ptr(small[0:1]), -1.0, ptr(tmp), 1.0, ptr(r))
+            if k == outer:     # the final residual H x - rho N x and N x, for the convergence record
+                _lib.call("cc_dot_f64", stream(), n, ptr(r), ptr(r), ptr(hist[outer:outer + 1]))
+                _lib.call("cc_dot_f64", stream(), n, ptr(tmp), ptr(tmp), ptr(hist[outer + 1:outer + 2]))
+                break
             _lib.call("cc_jd_correction_lp", stream(), self.Hlp.plan, f.n_env, f.d, ptr(f.Lt), ptr(f.W), ptr(f.Wt),
                       ptr(small[0:1]), ptr(xr), ptr(r), int(inner_iters), ptr(self._jd_work), ptr(t))
             for _ in range(2):
@@ -230,4 +243,4 @@ class MixedSolver:
                 _lib.call("cc_basis_combine_f64", stream(), k, n, ptr(U), ptr(c), ptr(tmp))
                 _lib.call("cc_axpby_f64", stream(), n, -1.0, ptr(tmp), 1.0, ptr(t))
             _lib.call("cc_normalize_f64", stream(), n, ptr(t), ptr(U[k]))
-        return xr, small[0:1]
+        return xr, small[0:1], hist

@@ -126,10 +126,12 @@ def _axpby(a, x, b, y):
 
 
 class _DenseInverse:
-    """x -> N^-1 x through W = L^-1 of the formed matrix: N^-1 = W^T W.  A normalization matrix
-    that is only numerically semi-definite (an environment right after a bandwidth increase) is
-    factored with escalating relative diagonal shifts 1e-14, 1e-12, 1e-10 (the reference's LU
-    inverts it unshifted)."""
+    """x -> N^-1 x for the formed normalization matrix (relaxOver's LU solve, utils.py:814-817).
+    A positive definite N goes through W = L^-1 of its Cholesky factor (N^-1 = W^T W).  One that is
+    only numerically semi-definite (an environment right after a bandwidth increase) is factored with
+    escalating relative diagonal shifts 1e-14, 1e-12, 1e-10.  An indefinite N (environments that are
+    Hermitian but not positive, as the reference's random systems) is inverted through its SVD,
+    N^-1 = V diag(1/s) U^T, which is the LU's inverse for any invertible N (n <= 256)."""
 
     def __init__(self, matrix):
         n = matrix.shape[0]
@@ -137,7 +139,7 @@ class _DenseInverse:
         trace = float(torch.diagonal(base).sum().item())
         work = empty((_lib.query("cc_linalg_work_size", n),))
         info = torch.zeros(1, dtype=torch.int32, device=base.device)
-        self.W = empty((n, n))
+        self.W, self.Ninv, self.n = empty((n, n)), None, n
         for shift in (0.0, 1e-14, 1e-12, 1e-10):
             L = base.clone()
             if shift:
@@ -146,18 +148,71 @@ class _DenseInverse:
             if int(info.item()) == 0:
                 break
         else:
-            raise ValueError("the normalization matrix is not positive definite (pivot {}); the HIP path "
-                             "factors N by Cholesky".format(int(info.item())))
+            if n > 256:
+                raise ValueError("the normalization matrix is not positive definite (pivot {}); the HIP path "
+                                 "factors N > 256 by Cholesky".format(int(info.item())))
+            self.Ninv = self._svd_inverse(base.contiguous(), n)
+            return
         self.Wt = empty(L.shape)
         _lib.call("cc_transpose_f64", stream(), n, ptr(self.W), ptr(self.Wt))
-        self.n = n
+
+    @staticmethod
+    def _svd_inverse(A, n):
+        U, S, Vt = empty((n, n)), empty((n,)), empty((n, n))
+        work = empty((_lib.query("cc_svd_work_size", 1, n, n),))
+        _lib.call("cc_svd_f64", stream(), 1, n, n, ptr(A), ptr(U), ptr(S), ptr(Vt), ptr(work))
+        Sinv, Us = empty((n,)), empty((n, n))
+        _lib.call("cc_unary_f64", stream(), 4, n, ptr(S), 0.0, ptr(Sinv))
+        _lib.call("cc_scale_rows_cols_f64", stream(), 1, n, n, ptr(U), ptr(Sinv), ptr(Us))      # U diag(1/s)
+        out = empty((n, n))
+        # N^-1 = Vt^T (U diag(1/s))^T
+        _lib.call("cc_gemm_f64", stream(), 1, n, n, n, 1.0, ptr(Vt), 0, 1, n, ptr(Us), 0, 1, n, 0.0, ptr(out), 0, n, 1)
+        return out
 
     def __call__(self, x):
-        t = torch.empty_like(x)
         y = torch.empty_like(x)
+        if self.Ninv is not None:
+            _lib.call("cc_skinny_f64", stream(), self.n, 0, ptr(self.Ninv), 1, ptr(x), ptr(y))
+            return y
+        t = torch.empty_like(x)
         _lib.call("cc_skinny_f64", stream(), self.n, 1, ptr(self.W), 1, ptr(x), ptr(t))
         _lib.call("cc_skinny_f64", stream(), self.n, 2, ptr(self.Wt), 1, ptr(t), ptr(y))
         return y
+
+
+def dense_lowest_eigenpair(H, N):
+    """Lowest eigenpair of the pencil (H, N) of formed n x n device matrices (n <= 256), for operators
+    without a fused kernel: N = L L^T (Cholesky with inverse W = L^-1, escalating shifts as
+    _DenseInverse), M = W H W^T symmetrised, and the one-sided Jacobi SVD of the positive semi-definite
+    M + s I with s = |M|_F >= the spectral radius, whose singular values are the eigenvalues of M plus s.
+    Returns (x, value) with x = W^T y N-normalised.  This is scipy.linalg.eigh(H, N)[0] of
+    minimizeExpectationUsingFullEigensolver (system/_2d.py:498-502)."""
+    n = H.shape[0]
+    if n > 256:
+        raise NotImplementedError("the dense generalized eigensolve handles n <= 256 (here n = {}); "
+                                  "larger operators have the fused kernel".format(n))
+    inv = _DenseInverse(N)
+    W, Wt = inv.W, inv.Wt
+    Hm = H.tensor().contiguous()
+    T, M = empty((n, n)), empty((n, n))
+    _lib.call("cc_gemm_f64", stream(), 1, n, n, n, 1.0, ptr(W), 0, n, 1, ptr(Hm), 0, n, 1, 0.0, ptr(T), 0, n, 1)
+    _lib.call("cc_gemm_f64", stream(), 1, n, n, n, 1.0, ptr(T), 0, n, 1, ptr(Wt), 0, n, 1, 0.0, ptr(M), 0, n, 1)
+    Mt = empty((n, n))
+    _lib.call("cc_transpose_f64", stream(), n, ptr(M), ptr(Mt))
+    _lib.call("cc_axpby_f64", stream(), n * n, 0.5, ptr(Mt), 0.5, ptr(M))
+    s = float(np.sqrt(max(_dot(M.reshape(-1), M.reshape(-1)), 0.0))) + 1.0
+    shift = empty((n, n))
+    _lib.call("cc_fill_f64", stream(), n * n, 0.0, ptr(shift))
+    _lib.call("cc_set_identity_f64", stream(), n, ptr(shift))
+    _lib.call("cc_axpby_f64", stream(), n * n, s, ptr(shift), 1.0, ptr(M))
+    U, S, Vt = empty((n, n)), empty((n,)), empty((n, n))
+    work = empty((_lib.query("cc_svd_work_size", 1, n, n),))
+    _lib.call("cc_svd_f64", stream(), 1, n, n, ptr(M), ptr(U), ptr(S), ptr(Vt), ptr(work))
+    y = Vt[n - 1].contiguous()
+    x = empty((n,))
+    _lib.call("cc_skinny_f64", stream(), n, 2, ptr(Wt), 1, ptr(y), ptr(x))
+    value = float(S[n - 1].item()) - s
+    return x, value
 
 
 def relax_over(initial, expectation_multiplier, normalization_multiplier=None, maximum_number_of_multiplications=None,

@@ -407,9 +407,14 @@ class System(BaseSystem):
         self.setStateCenter(HipData(x.reshape(self.state_center_data.shape)))
 
     def minimizeExpectationUsingFullEigensolver(self):
-        """The exact lowest generalized eigenpair (residual-controlled Lanczos run to 1e-12)."""
-        from ..solver import GroundStateSolver
+        """The exact lowest generalized eigenpair: residual-controlled Lanczos run to 1e-12 where the
+        operator has a fused kernel, else a dense generalized eigensolve of the formed H and N."""
+        from ..solver import GroundStateSolver, dense_lowest_eigenpair
         H, N = self.formExpectationAndNormalizationMultipliers()
+        if not H.operator.use_fused:
+            x, _ = dense_lowest_eigenpair(H.formMatrix(), N.formMatrix())
+            self.setStateCenter(HipData(x.reshape(self.state_center_data.shape)))
+            return
         x, theta, resid, iters = GroundStateSolver(H.operator, N.operator).solve(
             self.state_center_data, tol_residual=1e-12, tol_energy=0.0, max_iter=H.operator.n)
         self.setStateCenter(HipData(x.reshape(self.state_center_data.shape)))

@@ -219,6 +219,9 @@ int cc_stage3_lp_plan_create(void** out, int dtype, int n_a, const void* const* 
                              int64_t n_alpha, int64_t P, int64_t d, int groups, int split, float* work,
                              void* stream, const double* channel_scales);
 int cc_stage3_lp_plan_chunks(const void* plan, int* out);
+/* out = 1 when every term's coefficient matrix is diagonal or anti-diagonal (d = 2) and the plan runs the
+ * kernel's diagonal-form combine, 0 when some term has a full coefficient matrix. */
+int cc_stage3_lp_plan_diagonal_form(const void* plan, int* out);
 int cc_stage3_lp_plan_apply(void* stream, const void* plan, const double* x, double* y);
 int cc_stage3_lp_plan_destroy(void* plan);
 

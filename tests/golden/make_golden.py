@@ -1,8 +1,13 @@
-"""Writes tests/golden/energies.json: oracle ground-state energies (lowest generalized
-eigenvalue of the dense (H, N) from the oracle's stage 1/2/3, scipy.linalg.eigh) and
-matvec checksums for the seeded synthetic environments at the benchmark shapes.
+"""Writes the oracle's golden values for the GPU parity tests:
 
-Run from the repository root:  python tests/golden/make_golden.py
+* tests/golden/energies.json: ground-state energies (lowest generalized eigenvalue of the dense (H, N)
+  from the oracle's stage 1/2/3, scipy.linalg.eigh) and matvec checksums for the seeded synthetic
+  environments at the benchmark shapes;
+* tests/golden/batched_energies.json: the same ground-state energy for each of the 64 systems of the
+  batched workload (carcassonne_amd.batched.draw_specs(64, seed=0): (D, chi) uniform over
+  {(3,24), (4,32), (5,48), (6,64)}, h uniform in [2, 4], environment seed 1000 + k).
+
+Run from the repository root:  python tests/golden/make_golden.py [energies|batched]
 """
 import json
 import os
@@ -22,13 +27,18 @@ CASES = [(2, 8, "ising", 0), (4, 32, "ising", 0), (4, 32, "heisenberg", 0), (3, 
          (6, 64, "ising", 0), (6, 64, "heisenberg", 0)]
 
 
-def main():
+def ground_energy(D, chi, model, seed, h=3.0):
+    c, s, st = syn.make_environment(D, chi, model=model, seed=seed)
+    op = sparse2d.form_multipliers(*to_oracle(c, s, syn.operator_center(model, h=h)))
+    H, N = op.matrix_H(), op.matrix_N()
+    w = eigh(H, N, eigvals_only=True, subset_by_index=[0, 1])
+    return op, st, w
+
+
+def energies():
     out = {}
     for D, chi, model, seed in CASES:
-        c, s, st = syn.make_environment(D, chi, model=model, seed=seed)
-        op = sparse2d.form_multipliers(*to_oracle(c, s, syn.operator_center(model)))
-        H, N = op.matrix_H(), op.matrix_N()
-        w = eigh(H, N, eigvals_only=True, subset_by_index=[0, 1])
+        op, st, w = ground_energy(D, chi, model, seed)
         hv = op.apply_H(st).ravel()
         nv = op.apply_N(st).ravel()
         key = "D{}_chi{}_{}_seed{}".format(D, chi, model, seed)
@@ -42,5 +52,20 @@ def main():
         json.dump(out, f, indent=1, sort_keys=True)
 
 
+def batched():
+    from carcassonne_amd.batched import draw_specs
+    rows = []
+    for (D, chi, h, seed) in draw_specs(64, seed=0):
+        _, _, w = ground_energy(D, chi, "ising", seed, h=h)
+        rows.append(dict(D=D, chi=chi, h=h, seed=seed, ground_energy=float(w[0]), first_gap=float(w[1] - w[0])))
+        print(rows[-1], flush=True)
+    with open(os.path.join(ROOT, "tests", "golden", "batched_energies.json"), "w") as f:
+        json.dump(dict(specs="draw_specs(64, seed=0)", model="ising", systems=rows), f, indent=1)
+
+
 if __name__ == "__main__":
-    main()
+    which = sys.argv[1:] or ["energies", "batched"]
+    if "energies" in which:
+        energies()
+    if "batched" in which:
+        batched()

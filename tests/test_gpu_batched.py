@@ -21,6 +21,7 @@ def test_graph_batch_matches_plain_fp64(gpu):
         b.replay()
         got = b.energies()
         assert got == eager, (rep, got, eager)
+        assert len(b.convergence()) == len(specs)
         for g, r in zip(got, ref):
             assert abs(g - r) <= 1e-10 * abs(r), (g, r)
         b.reference_energies()          # eager work between replays

@@ -112,3 +112,32 @@ def test_lowprec_plan_runs_and_chunks_at_d6(gpu, model):
         print(model, "terms", len(H._A), "chunks", lp.chunks, "groups", lp.groups, "err", err)
         assert lp.chunks == (1 if model == "ising" else 2), (model, lp.chunks)
         assert err < 3e-3, (model, groups, err)
+
+
+@pytest.mark.parametrize("split", ["none", "vector", "both"])
+@pytest.mark.parametrize("D,chi", [(4, 16), (6, 12)])
+def test_lowprec_full_coefficient_terms(gpu, D, chi, split):
+    """A site operator with a full 2x2 coefficient matrix (a longitudinal plus transverse field,
+    O = -3 (X + 0.3 Z)) added to the Heisenberg model gives a plan that mixes diagonal (Z), anti-diagonal
+    (X, iY) and full terms, so the low-precision kernel takes its general-coefficient path (over two term
+    chunks at D=6): in every split mode it gives the FP64 fused result to the rounding tolerance."""
+    from carcassonne_amd import synthetic as syn
+    from carcassonne_amd.precision import LowPrecisionOperator
+    from carcassonne_amd.tensors._2d import sparse as PS
+    from tests.oracle_bridge import to_product
+    c, s, st = syn.make_environment(D, chi, model="heisenberg", seed=13)
+    op = syn.operator_center("heisenberg")
+    op[("O",)] = -3.0 * (syn.X + 0.3 * syn.Z)
+    H, _ = PS.formExpectationAndNormalizationMultipliers(*to_product(c, s, op))
+    H = H.operator
+    x = torch.from_numpy(st.ravel()).cuda()
+    y64 = torch.empty_like(x)
+    H.apply_flat(x, y64)
+    lp = LowPrecisionOperator(H, "f16", split=split)
+    y = torch.empty_like(x)
+    lp.apply_flat(x, y)
+    err = float((y - y64).norm() / y64.norm())
+    print(D, chi, split, "terms", len(H._A), "chunks", lp.chunks, "diag form", lp.diagonal_form, "err", err)
+    assert not lp.diagonal_form
+    assert lp.chunks == (2 if D == 6 else 1)
+    assert 0.0 < err < (3e-2 if split == "none" else 3e-3), (D, split, err)

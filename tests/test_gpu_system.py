@@ -43,27 +43,28 @@ def test_observables_match_oracle(gpu):
 
 
 def test_minimize_expectation_relaxover_matches_oracle(gpu):
-    """The reference's solve (relaxOver, k=3, tol 1e-7, 100 multiplications) on a grown ferromagnet
-    environment: same number of restarts and the same energy."""
-    from carcassonne_amd.data import HipData
-    from carcassonne_amd.system import System
-    from oracle.system import System as OSystem
-    Z = np.diag([1.0, -1.0])
-    X = np.array([[0.0, 1.0], [1.0, 0.0]])
-    osys = OSystem.newTrivialWithSimpleSparseOperator(O=-0.5 * X, OO_LR=[Z, -Z], OO_UD=[Z, -Z])
-    psys = System.newTrivialWithSimpleSparseOperator(O=HipData.fromArray(-0.5 * X),
-                                                     OO_LR=[HipData.Z, -HipData.Z], OO_UD=[HipData.Z, -HipData.Z])
-    for sys_ in (psys, osys):
-        for d in (0, 2, 1, 3):
-            sys_.increaseBandwidth(d % 2, by=1, do_as_much_as_possible=True) if False else None
+    """The reference's solve (relaxOver, k=3, tol 1e-7, 100 multiplications) on a grown random
+    environment with a D=2 center on both axes: the same tensors on both sides (the oracle's random
+    system, uploaded), five contractions, then the solve takes the same number of restarts and
+    multiplications as the oracle's (more than one restart) and lands on the same energy.  (A bandwidth
+    increase from D=1 is not used to grow the center: its new isometries come from SVDs of rank-one
+    matricizations, whose null-space columns are a free choice of each SVD.)"""
+    psys, osys = _pair(5, maximum_dimension=2)
     for d in (0, 1, 2, 3, 0):
         osys.contractTowards(d)
         psys.contractTowards(d)
+    assert tuple(psys.state_center_data.shape) == (2, 2, 2, 2, 2) == tuple(osys.state.shape)
+    e_p0, e_o0 = psys.computeExpectation(), osys.computeExpectation()
+    assert abs(e_p0 - e_o0) <= 1e-9 * abs(e_o0)
     osys.minimizeExpectation()
     psys.minimizeExpectation()
+    print("oracle", osys.last_solve, "device", psys.last_solve, flush=True)
+    assert osys.last_solve["restarts"] > 1
     assert psys.last_solve["restarts"] == osys.last_solve["restarts"]
+    assert psys.last_solve["mults"] == osys.last_solve["mults"]
     e_p, e_o = psys.computeExpectation(), osys.computeExpectation()
-    assert abs(e_p - e_o) <= 1e-9 * abs(e_o)
+    assert abs(e_p - e_o.real) <= 1e-9 * abs(e_o)
+    assert e_p < e_p0
 
 
 def test_gauge_moves_leave_expectations_invariant(gpu):
@@ -260,3 +261,39 @@ def test_operator_compression_below_the_term_count(gpu):
     gs.compressCornerTwoSiteOperatorTowards(0, 1, 2)
     e2, n2 = gs.computeExpectationAndNormalization()
     assert abs(n2 / n1 - 1) < 1e-10 and abs(e2 - e1) < 1e-9 * max(1, abs(e1))
+
+
+@pytest.mark.parametrize("walk,axes", [((0, 2, 1, 3), (0,)), ((1, 0, 3, 2, 1), (1,)), ((0, 1, 2), (0, 1))])
+def test_full_eigensolver_matches_oracle_eigh(gpu, walk, axes):
+    """minimizeExpectationUsingFullEigensolver (system/_2d.py:498-502) on transverse-field Ising systems
+    grown by bandwidth increases and contractions on the oracle's side, then uploaded: centers
+    (2,1,2,1,2) and (1,2,1,2,2) have no fused kernel (the dense generalized eigensolve runs), (2,2,2,2,2)
+    has one (Lanczos runs).  Both give the oracle's eigh(H, N) energy; so does a trivial D=1 system."""
+    from carcassonne_amd.data import HipData
+    from carcassonne_amd.system import System
+    from oracle.system import System as OSystem
+    from tests.oracle_bridge import system_to_product
+    Z = np.diag([1.0, -1.0])
+    X = np.array([[0.0, 1.0], [1.0, 0.0]])
+    osys = OSystem.newTrivialWithSimpleSparseOperator(O=-0.7 * X + 0.2 * Z, OO_LR=[Z, -Z], OO_UD=[Z, -Z],
+                                                      rng=np.random.default_rng(3))
+    for a in axes:
+        osys.increaseBandwidth(a, by=1, do_as_much_as_possible=True)
+        osys.setStateCenter(osys.state)
+    for d in walk:
+        osys.contractTowards(d)
+    psys = system_to_product(osys)
+    H, _ = psys.formExpectationAndNormalizationMultipliers()
+    print(walk, axes, "center", tuple(psys.state_center_data.shape), "fused", H.operator.use_fused, flush=True)
+    assert H.operator.use_fused == (len(axes) == 2)
+    for s in (psys, osys):
+        s.minimizeExpectationUsingFullEigensolver()
+    e_p, e_o = psys.computeExpectation(), osys.computeExpectation()
+    assert abs(e_p - e_o) <= 1e-10 * abs(e_o), (e_p, e_o)
+    osys = OSystem.newTrivialWithSimpleSparseOperator(O=-0.7 * X + 0.2 * Z, OO_LR=[Z, -Z], OO_UD=[Z, -Z])
+    psys = System.newTrivialWithSimpleSparseOperator(O=HipData.fromArray(-0.7 * X + 0.2 * Z),
+                                                     OO_LR=[HipData.Z, -HipData.Z], OO_UD=[HipData.Z, -HipData.Z])
+    for s in (psys, osys):
+        s.minimizeExpectationUsingFullEigensolver()
+    e_p, e_o = psys.computeExpectation(), osys.computeExpectation()
+    assert abs(e_p - e_o) <= 1e-12 * max(1.0, abs(e_o)), (e_p, e_o)
