@@ -55,7 +55,7 @@ def absorb_center_ss_into_side(direction, T, S, Sc=None):
     if Sc is None:
         Sc = S.conj()
     ks, bs = _state_subscripts(direction, "lmn", "LMN")
-    r = np.einsum("abcdefxy," + ks + "z," + bs + "z->albLcdmeMfnN", T, S, Sc)
+    r = np.einsum("abcdefxy," + ks + "z," + bs + "z->albLcdmeMfnN", T, S, Sc, optimize=True)
     return _merge(r, 2, 2, 1, 2, 2, 1, 1, 1)
 
 
@@ -64,7 +64,7 @@ def absorb_center_sos_into_side(direction, T, S, Op, Sc=None):
     if Sc is None:
         Sc = S.conj()
     ks, bs = _state_subscripts(direction, "lmn", "LMN")
-    r = np.einsum("abcdefxy," + ks + "z," + bs + "w,wz->albLcdmeMfnN", T, S, Sc, Op)
+    r = np.einsum("abcdefxy," + ks + "z," + bs + "w,wz->albLcdmeMfnN", T, S, Sc, Op, optimize=True)
     return _merge(r, 2, 2, 1, 2, 2, 1, 1, 1)
 
 

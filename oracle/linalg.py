@@ -104,7 +104,7 @@ def compressor_matrix(Lt, U, Rt):
     #  joins: L2-U*0, U^H1-R0, U^T1-R1, U*1-U^T0, L3-R2 ; groups ((L0,R3),(L1,U^H0))
     UH = U.conj().T
     UT = U.T
-    A = np.einsum("lapw,pm,nb,mc,bcwr->lran", Lt, Uc, UH, UT, Rt)
+    A = np.einsum("lapw,pm,nb,mc,bcwr->lran", Lt, Uc, UH, UT, Rt, optimize=True)
     l, r, a, n = A.shape
     return A.reshape(l * r, a * n)
 
