@@ -83,6 +83,8 @@ SIGNATURES = {
     "cc_mlp_adam": [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _f64, _f64, _f64, _f64, _i64],
     "cc_stage3_matvec_fused_f64": [_vp, _i32, _pvp, _i32, _pvp, _i32, _pi32, _pi32, _pi32, _i32, _pf64, _i64, _i64,
                                    _i64, _vp, _vp, _vp],
+    "cc_stage3_matvec_fused_f64_noalloc": [_vp, _i32, _pvp, _i32, _pvp, _i32, _pi32, _pi32, _pi32, _i32, _pf64, _i64,
+                                           _i64, _i64, _vp, _vp],
 }
 
 _lib = None

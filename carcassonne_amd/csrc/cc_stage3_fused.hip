@@ -14,6 +14,7 @@
 // writes one partial; a second kernel sums the partials in a fixed order.
 #include "cc_common.h"
 
+#include <mutex>
 #include <vector>
 
 namespace {
@@ -215,13 +216,14 @@ __global__ void __launch_bounds__(256) reduce_partials_kernel(int64_t groups, in
 }
 
 template <int P, int DP>
-int launch_fused(hipStream_t st, const FusedPlan& plan, const double* v, double* y, double* work, int groups) {
+int launch_fused(hipStream_t st, const FusedPlan& plan, const double* v, double* y, double* work, int groups,
+                 bool profile = true) {
   hipEvent_t e0 = nullptr;
-  int rc = cc_prof_start(st, CC_PROF_FUSED_F64, &e0);
+  int rc = profile ? cc_prof_start(st, CC_PROF_FUSED_F64, &e0) : CC_OK;
   if (rc != CC_OK) return rc;
   fused_f64_kernel<P, DP><<<groups, 256, 0, st>>>(plan, v, work);
   CC_LAUNCH_CHECK();
-  rc = cc_prof_stop(st, CC_PROF_FUSED_F64, e0);
+  rc = profile ? cc_prof_stop(st, CC_PROF_FUSED_F64, e0) : CC_OK;
   if (rc != CC_OK) return rc;
   const int64_t n = (int64_t)P * P * DP;
   reduce_partials_kernel<<<(unsigned)((n + 15) / 16), 256, 0, st>>>(groups, n, work, y);
@@ -362,6 +364,67 @@ extern "C" int cc_stage3_matvec_fused_f64(void* stream, int n_a, const double* c
     case 16: return launch_fused<16, 2>(st, plan, v, y, work, groups);
     case 25: return launch_fused<25, 2>(st, plan, v, y, work, groups);
     case 36: return launch_fused<36, 2>(st, plan, v, y, work, groups);
+  }
+  return CC_ERR_UNSUPPORTED;
+}
+
+// ---- the allocation-free entry point -----------------------------------------------------------
+// Scratch of cc_stage3_matvec_fused_f64_noalloc: the workgroup partial sums (groups x P^2 d doubles).
+// Every GPU the library runs on holds its own copy until the process ends.
+static __device__ __align__(256) double cc_noalloc_scratch[CC_NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+constexpr int kNoallocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+double* g_noalloc_scratch[kNoallocMaxDevices] = {};
+
+// The device address of cc_noalloc_scratch on the current device, looked up once per device (the host-side
+// name is only a stand-in for the symbol and is never passed to a kernel or a copy).
+int noalloc_scratch(double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) { cc_set_last_error(hipGetErrorString(e)); return CC_ERR_HIP; }
+  if (dev < 0 || dev >= kNoallocMaxDevices) { cc_set_last_error("noalloc: device index above 63"); return CC_ERR_UNSUPPORTED; }
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_scratch[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(cc_noalloc_scratch));
+    if (e != hipSuccess || !p) {
+      cc_set_last_error(e != hipSuccess ? hipGetErrorString(e) : "noalloc: null scratch address");
+      return CC_ERR_HIP;
+    }
+    g_noalloc_scratch[dev] = static_cast<double*>(p);
+  }
+  *out = g_noalloc_scratch[dev];
+  return CC_OK;
+}
+}  // namespace
+
+extern "C" int cc_stage3_matvec_fused_f64_noalloc(void* stream, int n_a, const double* const* A_packed, int n_b,
+                                                  const double* const* B_packed, int n_terms, const int* term_a,
+                                                  const int* term_b, const int* term_o, int n_ops,
+                                                  const double* host_ops, int64_t n_alpha, int64_t P, int64_t d,
+                                                  const double* v, double* y) {
+  if (!cc_stage3_fused_supported(P, d)) { cc_set_last_error("fused stage-3: shape not instantiated"); return CC_ERR_UNSUPPORTED; }
+  FusedPlan plan;
+  int rc = fill_plan(plan, n_a, A_packed, n_b, B_packed, n_terms, term_a, term_b, term_o, n_ops, host_ops, n_alpha, d);
+  if (rc) return rc;
+  const int groups = fused_groups(n_alpha);
+  if ((int64_t)groups * P * P * d > (int64_t)CC_NOALLOC_SCRATCH_DOUBLES) {
+    cc_set_last_error("noalloc stage-3: the partial sums need more than the 1048576-double (8 MiB) scratch; "
+                      "use cc_stage3_matvec_fused_f64 with a workspace");
+    return CC_ERR_ARG;
+  }
+  double* scratch = nullptr;
+  rc = noalloc_scratch(&scratch);
+  if (rc) return rc;
+  hipStream_t st = cc_stream(stream);
+  switch (P) {
+    case 4: return launch_fused<4, 2>(st, plan, v, y, scratch, groups, false);
+    case 9: return launch_fused<9, 2>(st, plan, v, y, scratch, groups, false);
+    case 16: return launch_fused<16, 2>(st, plan, v, y, scratch, groups, false);
+    case 25: return launch_fused<25, 2>(st, plan, v, y, scratch, groups, false);
+    case 36: return launch_fused<36, 2>(st, plan, v, y, scratch, groups, false);
   }
   return CC_ERR_UNSUPPORTED;
 }

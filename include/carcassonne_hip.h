@@ -136,6 +136,20 @@ int cc_stage3_matvec_fused_f64(void* stream, int n_a, const double* const* A_pac
                                int64_t n_alpha, int64_t P, int64_t d, const double* v, double* y,
                                double* work);
 
+/* cc_stage3_matvec_fused_f64 without any allocation and without a workspace argument: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  The workgroup partial sums live in a file-scope device array of
+ * CC_NOALLOC_SCRATCH_DOUBLES doubles (8 MiB) that every GPU the library runs on holds until the process ends;
+ * its address is looked up once per device on the first call made with that device current (the stream must
+ * belong to the current device).  Problems whose partial sums exceed it (groups * P^2 * d doubles, groups =
+ * min(1024, ceil(n_alpha / 16)); e.g. n_alpha > 6464 at P = 36) return CC_ERR_ARG.  All calls on one GPU share
+ * the array: calls that overlap in time on one GPU (different streams, or different host threads) race. */
+#define CC_NOALLOC_SCRATCH_DOUBLES 1048576
+int cc_stage3_matvec_fused_f64_noalloc(void* stream, int n_a, const double* const* A_packed, int n_b,
+                                       const double* const* B_packed, int n_terms, const int* term_a,
+                                       const int* term_b, const int* term_o, int n_ops, const double* host_ops,
+                                       int64_t n_alpha, int64_t P, int64_t d, const double* v, double* y);
+
 /* Persistent fused-operator plan: validated and copied once, applied many times (the
  * operator of one solve).  `work` (cc_stage3_fused_work_size doubles) stays caller-owned.
  * The plan object holding them is freed by cc_stage3_plan_destroy. */
