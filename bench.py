@@ -92,7 +92,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--form", choices=["both", "mixed", "fp64"], default="both")
+    ap.add_argument("--form", choices=["both", "mixed", "fp64", "absorb"], default="both",
+                    help="absorb: time the fused-bond sweep step (System.contractAndCompressTowards) instead")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first launch a plain copy kernel (unary_kernel, scale by 1) over 512 MiB three times: "
@@ -130,6 +131,29 @@ def main():
                       dst.data_ptr())
         synchronize()
         del src, dst
+
+    if args.form == "absorb":
+        from carcassonne_amd.system import System
+        system = System(corners, sides, x0, op_center)
+        directions = iter(range(10 ** 9))
+        step = lambda: system.contractAndCompressTowards(next(directions) % 4, WORKLOAD["chi"])
+        for _ in range(args.warmup):
+            step()
+        synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        e, n = system.computeExpectationAndNormalization()
+        print(json.dumps({"metric": "fused-bond sweep steps/s (absorb + compress) at D=6,chi=64", "value": 1e3 / ms,
+                          "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                          "ms_per_step": ms, "higher_is_better": True, "dtype": "f64",
+                          "data": "synthetic (seeded environment, carcassonne_amd.synthetic seed 0)",
+                          "config": {"workload": "contractAndCompressTowards on the 2D TFIM environment, D=6, "
+                                                 "chi=64, bonds cut back to 64", "D": 6, "chi": 64},
+                          "energy_after": e, "normalization_after": n}), flush=True)
+        return
 
     def run(fn, steps, warmup):
         for _ in range(warmup):

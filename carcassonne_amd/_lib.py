@@ -35,6 +35,9 @@ SIGNATURES = {
     "cc_scale_rows_cols_f64": [_vp, _i32, _i64, _i64, _vp, _vp, _vp],
     "cc_svd_work_size": [_i64, _i64, _i64, _pi64],
     "cc_svd_f64": [_vp, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
+    "cc_fused_truncation_work_size": [_i64, _i64, _i64, _pi64],
+    "cc_fused_truncation_f64": [_vp, _i64, _i64, _i64, _i64, _vp, _vp, _vp, _vp],
+    "cc_sign_normalize_columns_f64": [_vp, _i64, _i64, _vp],
     "cc_stage3_matvec_generic_work_size": [_i64, _i64, _i64, _i64, _pi64],
     "cc_stage3_matvec_generic_f64": [_vp, _i32, _pvp, _pvp, _pvp, _i64, _i64, _i64, _i64, _vp, _vp, _vp],
     "cc_stage3_fused_supported": [_i64, _i64],

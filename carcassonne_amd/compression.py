@@ -4,9 +4,11 @@
 Four alternating-least-squares sweeps: with the current isometry U the generated
 contractor forms A (l*r x old*new) such that A vec(X) = L (X U^H (x) conj) R; the new X
 solves the normal equations A^T A x = A^T b (b = vec(L R)), then U = polar(X).  The
-reference hands the normal equations to SciPy GMRES (relative tolerance 1e-5); here they
-are solved directly by a Cholesky factorization on the GPU with a 1e-14 relative diagonal
-shift, which returns the same least-squares solution without the GMRES truncation.
+reference hands the normal equations to SciPy GMRES (relative tolerance 1e-5); here the
+minimum-norm least-squares x comes from the eigen-decomposition of A^T A (one-sided Jacobi
+on the GPU) with eigenvalues below 1e-10 of the largest dropped, or, above 256 unknowns, a
+Cholesky factorization with a 1e-10 relative diagonal shift.  The start is the reference's
+random isometry, or (start="svd") a deterministic one, compressorStart.
 """
 import numpy as np
 import torch
@@ -30,10 +32,11 @@ _form = [
 ]
 
 
-def _solve_normal_equations(A, b, cutoff=1e-12):
+def _solve_normal_equations(A, b, cutoff=1e-10):
     """The minimum-norm least-squares x of A x = b: the normal equations A^T A x = A^T b solved with
-    the eigen-decomposition of A^T A (one-sided Jacobi on the GPU, eigenvalues below
-    cutoff * largest dropped) -- the solution GMRES from x0 = 0 converges to in the reference.
+    the eigen-decomposition of A^T A (one-sided Jacobi on the GPU, eigenvalues below cutoff * largest
+    dropped, i.e. singular values of A below 1e-5 of the largest, where the reference's GMRES stops at a
+    1e-5 relative residual).
     Systems with more than 256 unknowns use a Cholesky factorization with a 1e-10 relative shift."""
     At = A.join(1, 0)
     G = At.contractWith(A, (1,), (0,))
@@ -68,9 +71,29 @@ def _solve_normal_equations(A, b, cutoff=1e-12):
     return HipData(x)
 
 
+def compressorStart(L, new_dimension):
+    """A deterministic start for the compressor iteration (the reference draws a random isometry,
+    compression.py:35): the new_dimension leading left singular vectors of L matricized at its inward
+    ket leg, (old) x (l * old * w), from the eigenvectors of its Gram matrix, each column's sign fixed so
+    that its largest entry is positive; columns past the rank are completed with unit vectors and the
+    whole made an isometry."""
+    old = L.shape[1]
+    M = L.join(1, (0, 2, 3))
+    G = M.contractWith(M, (1,), (1,))
+    U, _, _ = G.svd(False)
+    U = U.tensor().contiguous()
+    _lib.call("cc_sign_normalize_columns_f64", stream(), old, old, ptr(U))   # the oracle's sign convention
+    U = HipData(U)
+    if new_dimension <= old:
+        return U[:, :new_dimension]
+    u = U.toArray()
+    return HipData.fromArray(np.hstack([u, np.identity(old)[:, :new_dimension - old]])).unitize()
+
+
 @prependDataContractor(*_form)
-def computeProductCompressor(formMatrix, L, R, new_dimension):
-    """Returns the compressor (new x old) for bond (L1, L2) = (R0, R1); compression.py:26-45."""
+def computeProductCompressor(formMatrix, L, R, new_dimension, start="random"):
+    """Returns the compressor (new x old) for bond (L1, L2) = (R0, R1); compression.py:26-45.
+    start="random" is the reference's random isometric start, start="svd" compressorStart."""
     if L.shape[1] != L.shape[2]:
         raise ValueError("left inward dimensions do not match (given " + str(L.shape) + ")")
     if R.shape[0] != R.shape[1]:
@@ -79,7 +102,10 @@ def computeProductCompressor(formMatrix, L, R, new_dimension):
         raise ValueError("left and right shapes are incompatible (given " + str(L.shape) + " and " + str(R.shape) + ")")
     old_dimension = L.shape[1]
     b = L.contractWith(R, (1, 2, 3), (0, 1, 2)).ravel()
-    compressor = HipData.newRandom(old_dimension, new_dimension).unitize()
+    if start == "svd":
+        compressor = compressorStart(L, new_dimension)
+    else:
+        compressor = HipData.newRandom(old_dimension, new_dimension).unitize()
     for _ in range(4):
         A = formMatrix(L, compressor.conj(), compressor.transpose().conj(), compressor.transpose(), R)
         x = _solve_normal_equations(A, b)
@@ -87,4 +113,4 @@ def computeProductCompressor(formMatrix, L, R, new_dimension):
     return compressor.transpose()
 
 
-__all__ = ["computeProductCompressor"]
+__all__ = ["computeProductCompressor", "compressorStart"]

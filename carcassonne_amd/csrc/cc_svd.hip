@@ -218,3 +218,91 @@ extern "C" int cc_svd_f64(void* stream, int64_t batch, int64_t m, int64_t n, con
   CC_LAUNCH_CHECK();
   return CC_OK;
 }
+
+// ---- truncation basis of a fused environment bond -----------------------------------------------------
+namespace {
+// Cs[(x, a, b)][j] = (C[(x, a, b)][j] + C[(x, b, a)][j]) / 2: the part of the bond matricization that is
+// symmetric under the ket <-> bra exchange of the absorbed center legs (a, b).
+__global__ void fused_symmetrize_kernel(int64_t nx, int64_t na, int64_t far, const double* __restrict__ C,
+                                        double* __restrict__ Cs) {
+  const int64_t total = nx * na * na * far;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = i % far, r = i / far, b = r % na, a = (r / na) % na, x = r / (na * na);
+    Cs[i] = 0.5 * (C[i] + C[((x * na + b) * na + a) * far + j]);
+  }
+}
+// out[r][c] = U[r][c] for c < k (U is m x n row-major)
+__global__ void leading_columns_kernel(int64_t m, int64_t n, int64_t k, const double* __restrict__ U, double* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m * k; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = U[(i / k) * n + i % k];
+}
+// One workgroup per column j of U (m x n row-major): the column is negated when its entry of largest
+// magnitude (the first such row) is negative, so that SVD bases computed by different routines agree.
+__global__ void sign_normalize_columns_kernel(int64_t m, int64_t n, double* __restrict__ U) {
+  __shared__ double best_v[256];
+  __shared__ int64_t best_i[256];
+  const int64_t j = blockIdx.x;
+  double bv = -1.0;
+  int64_t bi = m;
+  for (int64_t i = threadIdx.x; i < m; i += blockDim.x) {
+    const double a = fabs(U[i * n + j]);
+    if (a > bv) { bv = a; bi = i; }
+  }
+  best_v[threadIdx.x] = bv;
+  best_i[threadIdx.x] = bi;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      const double v2 = best_v[threadIdx.x + o];
+      const int64_t i2 = best_i[threadIdx.x + o];
+      if (v2 > best_v[threadIdx.x] || (v2 == best_v[threadIdx.x] && i2 < best_i[threadIdx.x])) {
+        best_v[threadIdx.x] = v2;
+        best_i[threadIdx.x] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  const int64_t ib = best_i[0];
+  if (ib < m && U[ib * n + j] < 0.0)
+    for (int64_t i = threadIdx.x; i < m; i += blockDim.x) U[i * n + j] = -U[i * n + j];
+}
+}  // namespace
+
+extern "C" int cc_sign_normalize_columns_f64(void* stream, int64_t m, int64_t n, double* U) {
+  if (m < 0 || n < 0) { cc_set_last_error("sign normalize: negative size"); return CC_ERR_ARG; }
+  if (m == 0 || n == 0) return CC_OK;
+  sign_normalize_columns_kernel<<<(unsigned)n, 256, 0, cc_stream(stream)>>>(m, n, U);
+  CC_LAUNCH_CHECK();
+  return CC_OK;
+}
+
+extern "C" int cc_fused_truncation_work_size(int64_t nx, int64_t na, int64_t far, int64_t* out) {
+  const int64_t m = nx * na * na;
+  int64_t svd = 0;
+  cc_svd_work_size(1, m, far, &svd);
+  *out = svd + 2 * m * far + far * far;
+  return CC_OK;
+}
+
+extern "C" int cc_fused_truncation_f64(void* stream, int64_t nx, int64_t na, int64_t far, int64_t new_dim,
+                                       const double* C, double* U_out, double* S_out, double* work) {
+  const int64_t m = nx * na * na;
+  if (nx < 1 || na < 1 || far < 1 || new_dim < 1) { cc_set_last_error("fused truncation: bad sizes"); return CC_ERR_ARG; }
+  if (far > SVD_MAX_N) { cc_set_last_error("fused truncation: far bond above 256"); return CC_ERR_UNSUPPORTED; }
+  if (m < far) { cc_set_last_error("fused truncation: the fused bond must be at least the far bond"); return CC_ERR_ARG; }
+  if (new_dim > far) { cc_set_last_error("fused truncation: new dimension above the rank bound (the far bond)"); return CC_ERR_ARG; }
+  hipStream_t st = cc_stream(stream);
+  int64_t svd_work = 0;
+  cc_svd_work_size(1, m, far, &svd_work);
+  double* Cs = work + svd_work;
+  double* U = Cs + m * far;
+  double* Vt = U + m * far;
+  const unsigned blocks = (unsigned)((m * far + 255) / 256 < 65535 ? (m * far + 255) / 256 : 65535);
+  fused_symmetrize_kernel<<<blocks, 256, 0, st>>>(nx, na, far, C, Cs);
+  CC_LAUNCH_CHECK();
+  svd_jacobi_kernel<<<1, SVD_THREADS, 0, st>>>(m, (int)far, Cs, U, S_out, Vt, work);
+  CC_LAUNCH_CHECK();
+  leading_columns_kernel<<<blocks, 256, 0, st>>>(m, far, new_dim, U, U_out);
+  CC_LAUNCH_CHECK();
+  return cc_sign_normalize_columns_f64(stream, m, new_dim, U_out);
+}

@@ -115,3 +115,26 @@ def make_environment(D, chi, d=2, model="ising", seed=0, energy=-1.0, local=0.3,
     state = rng.standard_normal((D, D, D, D, d))
     state /= np.linalg.norm(state)
     return corners, sides, state
+
+
+def make_double_layer_environment(D, c, d=2, seed=0, rank=4):
+    """Returns (corners, sides, state) of a seeded double-layer environment in the reference's per-copy
+    layout (corner (c, c, 1, c, c, 1), side (c, c, 1, c, c, 1, D, D); bond = (ket, bra, operator)) with the
+    Identity channel only: each corner is sum_r m_r (x) m_r and each side sum_r t_r (x) t_r over `rank`
+    single-layer tensors m_r (c, c) and t_r (c, c, D) with N(0,1) entries, each tensor scaled to unit
+    Frobenius norm.  N is then a sum of norms of single-layer networks -- positive semi-definite, and
+    definite for rank > 1 -- H is Hermitian, and the sweep's absorptions and per-copy compressions
+    (U (x) U*) keep both properties.  All draws from numpy PCG64(seed)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    corners, sides = [], []
+    for _ in range(4):
+        m = rng.standard_normal((rank, c, c))
+        C = np.einsum("rab,rcd->acbd", m, m)
+        corners.append({("I",): (C / np.linalg.norm(C)).reshape(c, c, 1, c, c, 1)})
+    for _ in range(4):
+        t = rng.standard_normal((rank, c, c, D))
+        T = np.einsum("rabs,rcdt->acbdst", t, t)
+        sides.append({("I",): (T / np.linalg.norm(T)).reshape(c, c, 1, c, c, 1, D, D)})
+    state = rng.standard_normal((D, D, D, D, d))
+    state /= np.linalg.norm(state)
+    return corners, sides, state

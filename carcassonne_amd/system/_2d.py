@@ -28,6 +28,39 @@ def sideFromCorner(corner_id, direction):
     return (corner_id + 1 - direction) % 4
 
 
+def fusedTruncationBasis(corner, bond_legs, far_legs, new_dimension):
+    """The (x D D) x k truncation basis of a grown fused corner bond, bond legs (x D, D, 1) with the old bond
+    x and the absorbed side's center ket / bra legs D, from the corner's Identity channel matricized bond x
+    far (cc_fused_truncation_f64); k = min(new_dimension, far)."""
+    from .. import _lib
+    from ..device import empty, ptr, stream
+    sh = corner.shape
+    b = [sh[j] for j in bond_legs]
+    D = b[1]
+    if b[2] != 1 or b[0] % D:
+        raise ValueError("fused truncation needs a grown fused bond (x D, D, 1), got {}".format(tuple(b)))
+    nx = b[0] // D
+    far = int(np.prod([sh[j] for j in far_legs]))
+    k = min(int(new_dimension), far)
+    C = corner.join(tuple(bond_legs), tuple(far_legs)).tensor().contiguous()
+    m = nx * D * D
+    U, S = empty((m, k)), empty((far,))
+    work = empty((_lib.query("cc_fused_truncation_work_size", nx, D, far),))
+    _lib.call("cc_fused_truncation_f64", stream(), nx, D, far, k, ptr(C), ptr(U), ptr(S), ptr(work))
+    return HipData(U)
+
+
+def _project_corner(t, bond_legs, basis):
+    """A corner channel with its fused bond legs (x D, D, 1) replaced by (k, 1, 1) through `basis`."""
+    sh = t.shape
+    k = basis.shape[1]
+    if tuple(bond_legs) == (3, 4, 5):
+        out = t.join((0, 1, 2), (3, 4, 5)).contractWith(basis, (1,), (0,))
+        return out.split(sh[0], sh[1], sh[2], k, 1, 1)
+    out = basis.contractWith(t.join((0, 1, 2), (3, 4, 5)), (0,), (0,))
+    return out.split(k, 1, 1, sh[3], sh[4], sh[5])
+
+
 class System(BaseSystem):
     # -- constructors ----------------------------------------------------------------
     @classmethod
@@ -97,11 +130,14 @@ class System(BaseSystem):
         self.solver = "relaxover"
         self.solver_options = {}
         self.last_solve = {}
+        # start of the state compressor's iteration: "random" (the reference's, compression.py:35) or "svd"
+        self.compressor_start = "random"
 
     def __copy__(self):
         s = System(list(self.corners), list(self.sides), self.state_center_data, dict(self.operator_center_tensor),
                    self.state_center_data_conj)
         s.solver, s.solver_options = self.solver, dict(self.solver_options)
+        s.compressor_start = self.compressor_start
         return s
 
     # -- checks (_2d.py:132-190) ---------------------------------------------------------
@@ -181,7 +217,7 @@ class System(BaseSystem):
     def compressCornerStateTowardsLeft(self, corner_id, new_dimension):
         side_joined = self.sides[L(corner_id)][Identity()].join((0, 1, 2, 6, 7), 3, 4, 5)
         corner_joined = self.corners[corner_id][Identity()].join(0, 1, 2, (3, 4, 5))
-        U = computeProductCompressor(side_joined, corner_joined, new_dimension)
+        U = computeProductCompressor(side_joined, corner_joined, new_dimension, start=self.compressor_start)
         self.sides[L(corner_id)] = mapOverSparseData(lambda t: t.absorbMatrixAt(3, U).absorbMatrixAt(4, U.conj()),
                                                      self.sides[L(corner_id)])
         self.corners[corner_id] = mapOverSparseData(lambda t: t.absorbMatrixAt(0, U.conj()).absorbMatrixAt(1, U),
@@ -190,7 +226,7 @@ class System(BaseSystem):
     def compressCornerStateTowardsRight(self, corner_id, new_dimension):
         corner_joined = self.corners[corner_id][Identity()].join((0, 1, 2), 3, 4, 5)
         side_joined = self.sides[corner_id][Identity()].join(0, 1, 2, (3, 4, 5, 6, 7))
-        U = computeProductCompressor(corner_joined, side_joined, new_dimension)
+        U = computeProductCompressor(corner_joined, side_joined, new_dimension, start=self.compressor_start)
         self.corners[corner_id] = mapOverSparseData(lambda t: t.absorbMatrixAt(3, U).absorbMatrixAt(4, U.conj()),
                                                     self.corners[corner_id])
         self.sides[corner_id] = mapOverSparseData(lambda t: t.absorbMatrixAt(0, U.conj()).absorbMatrixAt(1, U),
@@ -348,6 +384,35 @@ class System(BaseSystem):
         if self.just_increased_bandwidth:
             raise InvariantViolatedError("Contracting the current center would blow up the condition number of the "
                                          "normalization matrix;  optimize it or replace it first.")
+
+    # -- the sweep step on fused environment bonds ------------------------------------------------
+    def contractAndCompressTowards(self, direction, new_dimension):
+        """contractTowards(direction) followed by the compression of the two bonds it grows, for an
+        environment whose bonds are fused ((chi, 1, 1): ket, bra and operator legs in one index, the layout
+        of the synthetic environments): corners `direction` and R(direction) absorb their sides as in
+        contractUnnormalizedTowards (system/_2d.py:443-454); each grown corner bond (chi, D, D) is cut to a
+        truncation basis (cc_fused_truncation_f64: the leading ket<->bra-symmetric left singular vectors of
+        the corner's Identity channel, as compressCornerStateTowards takes its compressor from the Identity
+        channel, system/_2d.py:191-228); every channel of both corners is projected onto it; and side
+        `direction` absorbs the center already projected onto both bases
+        (absorbSparseCenterSOSIntoSideProjected), so the grown side, (chi D^2)^2 D^2 elements per channel,
+        never exists.  Bonds come back as (k, 1, 1) with k = min(new_dimension, chi).  With k equal to the
+        corner's rank bound chi the Identity-channel products of both bonds are kept exactly."""
+        from ..tensors._2d.sparse import absorbSparseCenterSOSIntoSideProjected
+        if self.just_increased_bandwidth:
+            raise InvariantViolatedError("Contracting the current center would blow up the condition number of the "
+                                         "normalization matrix;  optimize it or replace it first.")
+        i = direction
+        iso, _, den = self.state_center_data.normalizeAxis(O(i))
+        corner_l = absorbSparseSideIntoCornerFromLeft(self.corners[i], self.sides[L(i)])
+        corner_r = absorbSparseSideIntoCornerFromRight(self.corners[R(i)], self.sides[R(i)])
+        left = fusedTruncationBasis(corner_l[Identity()], (3, 4, 5), (0, 1, 2), new_dimension)
+        right = fusedTruncationBasis(corner_r[Identity()], (0, 1, 2), (3, 4, 5), new_dimension)
+        self.sides[i] = absorbSparseCenterSOSIntoSideProjected(i, self.sides[i], iso, self.operator_center_tensor,
+                                                               left, right, iso.conj())
+        self.corners[i] = mapOverSparseData(lambda t: _project_corner(t, (3, 4, 5), left), corner_l)
+        self.corners[R(i)] = mapOverSparseData(lambda t: _project_corner(t, (0, 1, 2), right), corner_r)
+        self.setStateCenter(self.state_center_data.normalizeAxis(i)[0].absorbMatrixAt(i, den))
 
     # -- multipliers (_2d.py:455-488) --------------------------------------------------------
     def formExpectationAndNormalizationMultipliers(self, operator_center_tensor=None):

@@ -49,6 +49,40 @@ def absorbDenseCenterSOSIntoSide(direction, side, state_center_data, operator_ce
     return _sos[direction](side, state_center_data, state_center_data_conj, operator_center_data)
 
 
+def absorbDenseCenterSOSIntoSideProjected(direction, side, state, operator, left, right, state_conj=None):
+    """absorbDenseCenterSOSIntoSide (operator=None: absorbDenseCenterSSIntoSide) followed by the projection
+    of both grown bonds onto fused truncation bases, without forming the grown side:
+
+        out[k, k', o, o'] = sum left[(x, a, b), k] right[(y, c, e), k'] T[x, y, i, i'] v[.., s] O[s', s] v*[.., s']
+
+    with the side's center legs (i, i') meeting the state's leg `direction`, (a, b) / (c, e) its legs L / R
+    (ket, bra) and (o, o') its leg O.  The side's bond legs are in fused form (x, 1, 1) and (y, 1, 1); the
+    result is (k, 1, 1, k', 1, 1, o, o').  Four GEMMs; the largest intermediate has x D^4 d k elements
+    (the grown side would have x y D^6)."""
+    i = direction
+    sh = side.shape
+    if sh[1] * sh[2] * sh[4] * sh[5] != 1:
+        raise ValueError("the projected absorption needs fused side bonds (x, 1, 1), got " + str(tuple(sh)))
+    if state_conj is None:
+        state_conj = state.conj()
+    x, y = sh[0], sh[3]
+    T = side.join(0, 3, 6, 7)                                    # (x, y, i, i')
+    v = state.join(i, L(i), R(i), O(i), 4)                       # (i, a, c, o, s)
+    vb = state_conj.join(i, L(i), R(i), O(i), 4)                 # (i', b, e, o', s')
+    if operator is not None:                                     # (O v)[.., s'] = sum_s O[s', s] v[.., s]
+        v = operator.contractWith(v, (1,), (4,)).join(1, 2, 3, 4, 0)
+    D_a, D_c = v.shape[1], v.shape[2]
+    k, k2 = left.shape[1], right.shape[1]
+    lt = left.split(x, D_a, D_a, k)                              # (x, a, b, k)
+    rt = right.split(y, D_c, D_c, k2)                            # (y, c, e, k')
+    X1 = lt.contractWith(T, (0,), (0,))                          # (a, b, k, y, i, i')
+    X2 = X1.contractWith(v, (4, 0), (0, 1))                      # (b, k, y, i', c, o, s)
+    X3 = X2.contractWith(vb, (3, 0, 6), (0, 1, 4))               # (k, y, c, o, e, o')
+    out = X3.contractWith(rt, (1, 2, 4), (0, 1, 2))              # (k, o, o', k')
+    o = out.shape[1]
+    return out.join(0, 3, 1, 2).split(k, 1, 1, k2, 1, 1, o, o)
+
+
 # corner C(3,4,5) <-> side T(0,1,2) -> ((C0,C1,C2), (T3,T4,T5), T6, T7)          dense.py:96-99
 formNormalizationStage1 = formDataContractor(
     [Join(0, [3, 4, 5], 1, [0, 1, 2])],
@@ -103,6 +137,6 @@ def formNormalizationSubmatrix(corners, sides):
 
 
 __all__ = ["absorbDenseSideIntoCornerFromLeft", "absorbDenseSideIntoCornerFromRight", "absorbDenseCenterSSIntoSide",
-           "absorbDenseCenterSOSIntoSide", "formDenseStage3", "formDenseStage3_formMatrix_and_cost",
+           "absorbDenseCenterSOSIntoSide", "absorbDenseCenterSOSIntoSideProjected", "formDenseStage3", "formDenseStage3_formMatrix_and_cost",
            "formDenseStage3_multiply_and_cost", "formNormalizationMultiplier", "formNormalizationStage1",
            "formNormalizationStage2", "formNormalizationStage3", "formNormalizationSubmatrix"]

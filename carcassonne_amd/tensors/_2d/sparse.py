@@ -7,7 +7,8 @@ matvec of the whole expectation multiplier is a single HIP call.
 import itertools
 
 from .dense import *  # noqa: F401,F403
-from .dense import (absorbDenseCenterSOSIntoSide, absorbDenseCenterSSIntoSide, absorbDenseSideIntoCornerFromLeft,
+from .dense import (absorbDenseCenterSOSIntoSide, absorbDenseCenterSOSIntoSideProjected, absorbDenseCenterSSIntoSide,
+                    absorbDenseSideIntoCornerFromLeft,
                     absorbDenseSideIntoCornerFromRight, formNormalizationStage1, formNormalizationStage2)
 from ...sparse import (Complete, Identity, OneSiteOperator, TwoSiteOperator, TwoSiteOperatorCompressed,
                        addStandardCompleteAndIdentityTerms, contractSparseTensors, getInformationFromOperatorCenter)
@@ -42,6 +43,20 @@ def absorbSparseSideIntoCornerFromRight(corner, side):
     return contractSparseTensors(terms, corner, side)
 
 
+def _center_into_side_terms(direction, ss, sos):
+    """The rule table of absorbSparseCenterSOSIntoSide (sparse.py:36-58): (side tag, operator tag) ->
+    (result tag, dense contractor) with ss for the identity-operator pairs and sos for the others."""
+    return {
+        (Identity, Identity): lambda s, c: (Identity(), ss),
+        (Complete, Identity): lambda s, c: (Complete(), ss),
+        (Identity, OneSiteOperator): lambda s, c: (Complete(), sos),
+        (TwoSiteOperator, Identity): lambda s, c: (s.matchesCenterIdentity(), ss),
+        (Identity, TwoSiteOperator): lambda s, c: (c.matchesSideIdentityOutward(direction), sos),
+        (TwoSiteOperator, TwoSiteOperator): lambda s, c: (s.matchesCenter(direction, c), sos),
+        (TwoSiteOperatorCompressed, Identity): lambda s, c: (s, ss),
+    }
+
+
 def absorbSparseCenterSOSIntoSide(direction, side, state_center_data, operator_center_data, state_center_data_conj=None):
     """sparse.py:36-58 (handler arguments: side tag, operator tag)."""
     if state_center_data_conj is None:
@@ -52,16 +67,25 @@ def absorbSparseCenterSOSIntoSide(direction, side, state_center_data, operator_c
 
     def sos(side_data, op_data):
         return absorbDenseCenterSOSIntoSide(direction, side_data, state_center_data, op_data, state_center_data_conj)
-    terms = {
-        (Identity, Identity): lambda s, c: (Identity(), ss),
-        (Complete, Identity): lambda s, c: (Complete(), ss),
-        (Identity, OneSiteOperator): lambda s, c: (Complete(), sos),
-        (TwoSiteOperator, Identity): lambda s, c: (s.matchesCenterIdentity(), ss),
-        (Identity, TwoSiteOperator): lambda s, c: (c.matchesSideIdentityOutward(direction), sos),
-        (TwoSiteOperator, TwoSiteOperator): lambda s, c: (s.matchesCenter(direction, c), sos),
-        (TwoSiteOperatorCompressed, Identity): lambda s, c: (s, ss),
-    }
-    return contractSparseTensors(terms, side, operator_center_data)
+    return contractSparseTensors(_center_into_side_terms(direction, ss, sos), side, operator_center_data)
+
+
+def absorbSparseCenterSOSIntoSideProjected(direction, side, state_center_data, operator_center_data, left, right,
+                                           state_center_data_conj=None):
+    """absorbSparseCenterSOSIntoSide with every result channel projected onto the fused truncation bases
+    `left` (the bond to corner `direction`) and `right` (the bond to corner R(direction)) as it is formed
+    (absorbDenseCenterSOSIntoSideProjected): the same rule table, no grown side in memory."""
+    if state_center_data_conj is None:
+        state_center_data_conj = state_center_data.conj()
+
+    def ss(side_data, _):
+        return absorbDenseCenterSOSIntoSideProjected(direction, side_data, state_center_data, None, left, right,
+                                                     state_center_data_conj)
+
+    def sos(side_data, op_data):
+        return absorbDenseCenterSOSIntoSideProjected(direction, side_data, state_center_data, op_data, left, right,
+                                                     state_center_data_conj)
+    return contractSparseTensors(_center_into_side_terms(direction, ss, sos), side, operator_center_data)
 
 
 def formExpectationStage1(corner, side):

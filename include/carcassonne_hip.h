@@ -99,6 +99,25 @@ int cc_svd_work_size(int64_t batch, int64_t m, int64_t n, int64_t* out);
 int cc_svd_f64(void* stream, int64_t batch, int64_t m, int64_t n, const double* A,
                double* U, double* S, double* Vt, double* work);
 
+/* Truncation basis of a fused environment bond (the state compression of compressCornerStateTowards,
+ * system/_2d.py:191-228 with computeProductCompressor, compression.py:26-45, for a bond whose ket, bra and
+ * operator legs are fused into one index).  C is the corner's Identity channel matricized as
+ * (x, a, b) x far: the fused bond (x = the corner's old bond, a / b = the absorbed side's center ket / bra
+ * legs) against the corner's far bond (far <= 256, x a^2 >= far).  The rows are symmetrized under the
+ * a <-> b exchange (so the projected environment keeps its ket <-> bra symmetry) and U_out (x a^2 by
+ * new_dim, orthonormal columns, signs fixed by cc_sign_normalize_columns_f64) receives the leading new_dim
+ * left singular vectors, S_out (far) all
+ * singular values, descending.  new_dim <= far: the corner's rank bound, at which the Identity-channel
+ * product of corner and side is kept exactly.  `work` holds cc_fused_truncation_work_size doubles. */
+int cc_fused_truncation_work_size(int64_t nx, int64_t na, int64_t far, int64_t* out);
+/* Fixes the sign freedom of singular vectors: each column of the m x n row-major U is negated when its entry of
+ * largest magnitude (the first such row) is negative.  Applied to the bases of cc_fused_truncation_f64 and to
+ * the deterministic compressor start, so that the GPU and the oracle take the same gauge (the absorptions that
+ * follow identify the two bonds of a side, so the gauge is not free there). */
+int cc_sign_normalize_columns_f64(void* stream, int64_t m, int64_t n, double* U);
+int cc_fused_truncation_f64(void* stream, int64_t nx, int64_t na, int64_t far, int64_t new_dim, const double* C,
+                            double* U_out, double* S_out, double* work);
+
 /* ---- stage 3: the center-site effective-Hamiltonian matvec ----------------------
  * Y = sum_t (A_{a(t)} (x) B_{b(t)} (x) O_{o(t)}) v, summed over alpha, with
  *   A_a[alpha][p'][p] (P x P slabs), B_b[alpha][q'][q] (Q x Q slabs), v[p][q][s] (P,Q,d),

@@ -109,10 +109,49 @@ def compressor_matrix(Lt, U, Rt):
     return A.reshape(l * r, a * n)
 
 
-def product_compressor(Lt, Rt, new_dimension, rng, sweeps=4):
+def sign_normalize_columns(U):
+    """Each column negated when its entry of largest magnitude (the first such row) is negative."""
+    U = np.array(U, copy=True)
+    for j in range(U.shape[1]):
+        i = int(np.argmax(np.abs(U[:, j])))
+        if U[i, j].real < 0:
+            U[:, j] = -U[:, j]
+    return U
+
+
+def compressor_start(Lt, new_dimension):
+    """A deterministic start for the compressor iteration (the reference draws a random isometry,
+    compression.py:35): the new_dimension leading left singular vectors of L matricized at its
+    inward ket leg, (old) x (l * old * w) -- the ket subspace the left tensor uses most -- each with
+    the sign that makes its largest entry positive."""
+    old = Lt.shape[1]
+    m = np.moveaxis(Lt, 1, 0).reshape(old, -1)
+    u, _, _ = np.linalg.svd(m, full_matrices=False)
+    u = sign_normalize_columns(u)
+    if u.shape[1] >= new_dimension:
+        return u[:, :new_dimension]
+    return unitize(np.hstack([u, np.identity(old)[:, :new_dimension - u.shape[1]]]))
+
+
+def min_norm_solution(A, b, cutoff=1e-10):
+    """The minimum-norm least-squares x of A x = b from the normal equations: the eigenpairs of A^H A with
+    eigenvalues above cutoff * the largest (singular values above 1e-5 of the largest, as far as the
+    reference's GMRES, relative tolerance 1e-5, resolves them)."""
+    G = A.conj().T @ A
+    w, V = np.linalg.eigh(G)
+    keep = w > cutoff * max(w[-1], 1e-300)
+    if not np.any(keep):
+        return np.zeros(A.shape[1], dtype=np.result_type(A, b))
+    Vk = V[:, keep]
+    return Vk @ ((Vk.conj().T @ (A.conj().T @ b)) / w[keep])
+
+
+def product_compressor(Lt, Rt, new_dimension, rng, sweeps=4, start="random"):
     """computeProductCompressor (compression.py:26-45).  The normal equations
-    A^H A x = A^H b are solved exactly (lstsq) where the reference calls GMRES with
-    SciPy defaults; both target the same least-squares x.  Returns U^T (new x old)."""
+    A^H A x = A^H b are solved for the minimum-norm least-squares x with singular values below 1e-5
+    of the largest dropped (min_norm_solution), where the reference calls GMRES (relative tolerance
+    1e-5) on them.  start="random" draws the reference's random isometric start; start="svd" uses
+    compressor_start.  Returns U^T (new x old)."""
     if Lt.shape[1] != Lt.shape[2]:
         raise ValueError("left inward dimensions do not match (given " + str(Lt.shape) + ")")
     if Rt.shape[0] != Rt.shape[1]:
@@ -121,10 +160,10 @@ def product_compressor(Lt, Rt, new_dimension, rng, sweeps=4):
         raise ValueError("left and right shapes are incompatible")
     old = Lt.shape[1]
     b = np.tensordot(Lt, Rt, ((1, 2, 3), (0, 1, 2))).ravel()
-    U = unitize(rng.standard_normal((old, new_dimension)))
+    U = compressor_start(Lt, new_dimension) if start == "svd" else unitize(rng.standard_normal((old, new_dimension)))
     for _ in range(sweeps):
         A = compressor_matrix(Lt, U, Rt)
-        x = np.linalg.lstsq(A, b, rcond=None)[0]
+        x = min_norm_solution(A, b)
         U = unitize(x.reshape(old, new_dimension))
     return U.T
 

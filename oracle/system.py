@@ -51,6 +51,7 @@ class System:
         self.policies = {n: None for n in self.POLICY_NAMES}
         self.solver = "relaxover"   # or "eigh"
         self.last_solve = {}
+        self.compressor_start = "random"   # or "svd" (linalg.compressor_start)
 
     # -- constructors ---------------------------------------------------------------
     @classmethod
@@ -150,14 +151,14 @@ class System:
     def compressCornerStateTowardsLeft(self, c, new_dimension):
         side = linalg.join(self.sides[L(c)][Identity()], (0, 1, 2, 6, 7), 3, 4, 5)
         corner = linalg.join(self.corners[c][Identity()], 0, 1, 2, (3, 4, 5))
-        U = linalg.product_compressor(side, corner, new_dimension, self.rng)
+        U = linalg.product_compressor(side, corner, new_dimension, self.rng, start=self.compressor_start)
         self.sides[L(c)] = _map(lambda t: linalg.absorb_matrix_at(linalg.absorb_matrix_at(t, 3, U), 4, U.conj()), self.sides[L(c)])
         self.corners[c] = _map(lambda t: linalg.absorb_matrix_at(linalg.absorb_matrix_at(t, 0, U.conj()), 1, U), self.corners[c])
 
     def compressCornerStateTowardsRight(self, c, new_dimension):
         corner = linalg.join(self.corners[c][Identity()], (0, 1, 2), 3, 4, 5)
         side = linalg.join(self.sides[c][Identity()], 0, 1, 2, (3, 4, 5, 6, 7))
-        U = linalg.product_compressor(corner, side, new_dimension, self.rng)
+        U = linalg.product_compressor(corner, side, new_dimension, self.rng, start=self.compressor_start)
         self.corners[c] = _map(lambda t: linalg.absorb_matrix_at(linalg.absorb_matrix_at(t, 3, U), 4, U.conj()), self.corners[c])
         self.sides[c] = _map(lambda t: linalg.absorb_matrix_at(linalg.absorb_matrix_at(t, 0, U.conj()), 1, U), self.sides[c])
 

@@ -1,0 +1,89 @@
+"""GPU parity of the sweep step against the oracle.
+
+* The fused-bond step (System.contractAndCompressTowards: absorptions, fused truncation basis from
+  cc_fused_truncation_f64, the side absorption projected as it is formed) against the oracle's
+  contractTowards + fused projection (oracle/fused.py), on the synthetic fused environments: energy and
+  normalization after every step, at the rank bound and below it.
+* The plain CPU workload of the benchmark list: the 2D transverse-field Ising model (h = 3, d = 2) on a
+  seeded double-layer environment with center bond D = 2 and per-copy environment bond 8, 20 sweep steps
+  of contract -> compress every corner-side bond to 8 (the reference's per-copy compressor, deterministic
+  start) -> relaxOver solve, the reference's own step (system/_2d.py:435-497, policies.py:80-113).  Every
+  step of the GPU is run on the oracle's pre-step system (uploaded), so that each step is compared on
+  identical inputs: energy after the contraction (1e-9), after the compression and after the solve (1e-6:
+  the compressor's least-squares problems have condition numbers near 1e9, inherited from the
+  reference's algorithm), and the solve's restarts where the compressed inputs agree to 1e-9.  The oracle's one-site energy estimates over the last steps scatter around -3.1
+  (h = 3 is close to the 2D critical field 3.04, where the 2D TFIM energy per site is about -3.1).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("D,chi,model,new", [(2, 8, "ising", 8), (3, 6, "heisenberg", 6), (2, 8, "ising", 6)])
+def test_fused_step_matches_oracle(gpu, D, chi, model, new):
+    from carcassonne_amd import synthetic as syn
+    from oracle import fused
+    from oracle.system import System as OSystem
+    from tests.oracle_bridge import to_oracle, to_product
+    from carcassonne_amd.system import System
+    c, s, st = syn.make_environment(D, chi, model=model, seed=4)
+    op = syn.operator_center(model)
+    osys = OSystem(*to_oracle(c, s, op)[:2], st, to_oracle(c, s, op)[2])
+    pc, ps, po = to_product(c, s, op)
+    from carcassonne_amd.data import HipData
+    psys = System(pc, ps, HipData.fromArray(st), po)
+    for step, direction in enumerate((0, 1, 2, 3, 1)):
+        fused.contract_and_compress_towards(osys, direction, new)
+        psys.contractAndCompressTowards(direction, new)
+        e_o, n_o = osys.computeExpectationAndNormalization()
+        e_p, n_p = psys.computeExpectationAndNormalization()
+        print(D, chi, model, new, "step", step, "E", e_p, e_o, "N", n_p, n_o, flush=True)
+        assert abs(n_p - n_o) <= 1e-9 * abs(n_o), (step, n_p, n_o)
+        assert abs(e_p - e_o) <= 1e-9 * abs(e_o), (step, e_p, e_o)
+        assert all(t.shape[0] == min(new, chi) for t in psys.sides[direction].values())
+
+
+def test_plain_cpu_workload_sweep_steps_match_oracle(gpu):
+    from carcassonne_amd import synthetic as syn
+    from carcassonne_amd.data import HipData
+    from oracle.system import System as OSystem
+    from oracle.tags import make_simple_sparse_operator
+    from tests.oracle_bridge import system_to_product, to_oracle
+    X = np.array([[0.0, 1.0], [1.0, 0.0]])
+    Z = np.diag([1.0, -1.0])
+    c, s, st = syn.make_double_layer_environment(2, 8, seed=0)
+    oc, os_, _ = to_oracle(c, s, {})
+    osys = OSystem(oc, os_, st, make_simple_sparse_operator(O=-3.0 * X, OO_UD=[Z, -Z], OO_LR=[Z, -Z]),
+                   rng=np.random.default_rng(1))
+    osys.compressor_start = "svd"
+    per_site = []
+    for step in range(20):
+        psys = system_to_product(osys)
+        psys.compressor_start = "svd"
+        rows = []
+        for sys_ in (osys, psys):
+            sys_.contractTowards(step % 4)
+            e1 = sys_.computeExpectation()
+            for corner_id in range(4):
+                for direction in range(2):
+                    sys_.compressCornerStateTowards(corner_id, direction, 8)
+            e2 = sys_.computeExpectation()
+            sys_.minimizeExpectation()
+            rows.append((complex(e1).real, complex(e2).real, complex(sys_.computeExpectation()).real,
+                         sys_.last_solve["restarts"]))
+        (o1, o2, o3, orst), (p1, p2, p3, prst) = rows
+        site = complex(osys.computeOneSiteExpectation()).real
+        per_site.append(site)
+        print("step", step, "oracle", rows[0], "device", rows[1], "per site", site, flush=True)
+        assert abs(p1 - o1) <= 1e-9 * abs(o1), ("contract", step, p1, o1)
+        # the compressor's normal equations are ill-conditioned (kept eigenvalues down to ~1e-9 of the
+        # largest): a 1e-14 relative change of its input moves the compressed energy by ~2e-8 on the CPU
+        # alone, so the compressed and solved energies are compared to 1e-6
+        assert abs(p2 - o2) <= 1e-6 * abs(o2), ("compress", step, p2, o2)
+        if abs(p2 - o2) <= 1e-9 * abs(o2):
+            assert prst == orst, ("restarts", step, prst, orst)
+        assert abs(p3 - o3) <= 1e-6 * abs(o3), ("solve", step, p3, o3)
+        assert abs(complex(psys.computeOneSiteExpectation()).real - site) <= 1e-6 * abs(site)
+    # a loose physical check of the trajectory (the one-site estimates fluctuate from step to step)
+    assert -3.4 < float(np.mean(per_site[-8:])) < -2.8, per_site
