@@ -231,11 +231,6 @@ __global__ void fused_symmetrize_kernel(int64_t nx, int64_t na, int64_t far, con
     Cs[i] = 0.5 * (C[i] + C[((x * na + b) * na + a) * far + j]);
   }
 }
-// out[r][c] = U[r][c] for c < k (U is m x n row-major)
-__global__ void leading_columns_kernel(int64_t m, int64_t n, int64_t k, const double* __restrict__ U, double* __restrict__ out) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m * k; i += (int64_t)gridDim.x * blockDim.x)
-    out[i] = U[(i / k) * n + i % k];
-}
 // One workgroup per column j of U (m x n row-major): the column is negated when its entry of largest
 // magnitude (the first such row) is negative, so that SVD bases computed by different routines agree.
 __global__ void sign_normalize_columns_kernel(int64_t m, int64_t n, double* __restrict__ U) {
@@ -276,14 +271,32 @@ extern "C" int cc_sign_normalize_columns_f64(void* stream, int64_t m, int64_t n,
   return CC_OK;
 }
 
+namespace {
+// W[i][j] = V[i][j] / s_j with V = Vt^T and s_j = sqrt(lambda_j) the singular values of the matrix whose Gram
+// eigenvalues lambda (descending) and vectors Vt came from the SVD of the Gram matrix; columns whose s_j is at
+// most 1e-7 s_0 (below what the Gram matrix resolves) are zero.  S_out receives every s_j.
+__global__ void fused_gram_scale_kernel(int64_t far, int64_t k, const double* __restrict__ lam, const double* __restrict__ Vt,
+                                        double* __restrict__ W, double* __restrict__ S_out) {
+  const double s0 = sqrt(fmax(lam[0], 0.0));
+  for (int64_t e = threadIdx.x; e < far * k; e += blockDim.x) {
+    const int64_t i = e / k, j = e % k;
+    const double sj = sqrt(fmax(lam[j], 0.0));
+    W[e] = (sj > 1e-7 * s0 && sj > 0.0) ? Vt[j * far + i] / sj : 0.0;
+  }
+  for (int64_t j = threadIdx.x; j < far; j += blockDim.x) S_out[j] = sqrt(fmax(lam[j], 0.0));
+}
+}  // namespace
+
 extern "C" int cc_fused_truncation_work_size(int64_t nx, int64_t na, int64_t far, int64_t* out) {
   const int64_t m = nx * na * na;
   int64_t svd = 0;
-  cc_svd_work_size(1, m, far, &svd);
-  *out = svd + 2 * m * far + far * far;
+  cc_svd_work_size(1, far, far, &svd);
+  *out = svd + m * far + 4 * far * far + far;
   return CC_OK;
 }
 
+// The leading left singular vectors of the symmetrized C (m x far, m >= far) from its Gram matrix:
+// G = Cs^T Cs = V diag(lambda) V^T (one-sided Jacobi on the far x far G), U = Cs V diag(1/sqrt(lambda)).
 extern "C" int cc_fused_truncation_f64(void* stream, int64_t nx, int64_t na, int64_t far, int64_t new_dim,
                                        const double* C, double* U_out, double* S_out, double* work) {
   const int64_t m = nx * na * na;
@@ -293,16 +306,23 @@ extern "C" int cc_fused_truncation_f64(void* stream, int64_t nx, int64_t na, int
   if (new_dim > far) { cc_set_last_error("fused truncation: new dimension above the rank bound (the far bond)"); return CC_ERR_ARG; }
   hipStream_t st = cc_stream(stream);
   int64_t svd_work = 0;
-  cc_svd_work_size(1, m, far, &svd_work);
+  cc_svd_work_size(1, far, far, &svd_work);
   double* Cs = work + svd_work;
-  double* U = Cs + m * far;
-  double* Vt = U + m * far;
+  double* G = Cs + m * far;
+  double* Ug = G + far * far;
+  double* Vt = Ug + far * far;
+  double* W = Vt + far * far;
+  double* lam = W + far * far;
   const unsigned blocks = (unsigned)((m * far + 255) / 256 < 65535 ? (m * far + 255) / 256 : 65535);
   fused_symmetrize_kernel<<<blocks, 256, 0, st>>>(nx, na, far, C, Cs);
   CC_LAUNCH_CHECK();
-  svd_jacobi_kernel<<<1, SVD_THREADS, 0, st>>>(m, (int)far, Cs, U, S_out, Vt, work);
+  int rc = cc_gemm_f64(stream, 1, far, far, m, 1.0, Cs, 0, 1, far, Cs, 0, far, 1, 0.0, G, 0, far, 1);
+  if (rc) return rc;
+  svd_jacobi_kernel<<<1, SVD_THREADS, 0, st>>>(far, (int)far, G, Ug, lam, Vt, work);
   CC_LAUNCH_CHECK();
-  leading_columns_kernel<<<blocks, 256, 0, st>>>(m, far, new_dim, U, U_out);
+  fused_gram_scale_kernel<<<1, 256, 0, st>>>(far, new_dim, lam, Vt, W, S_out);
   CC_LAUNCH_CHECK();
+  rc = cc_gemm_f64(stream, 1, m, new_dim, far, 1.0, Cs, 0, far, 1, W, 0, new_dim, 1, 0.0, U_out, 0, new_dim, 1);
+  if (rc) return rc;
   return cc_sign_normalize_columns_f64(stream, m, new_dim, U_out);
 }

@@ -103,6 +103,13 @@ class LowPrecisionOperator:
                 pass
 
 
+def _schedule(inner_iters, k):
+    """CG steps of the k-th correction (0-based): a fixed count, or a list whose last entry repeats."""
+    if isinstance(inner_iters, (list, tuple)):
+        return int(inner_iters[min(k, len(inner_iters) - 1)])
+    return int(inner_iters)
+
+
 class MixedSolver:
     """Lowest eigenpair of (H, N) to FP64 accuracy with low-precision matvecs doing the bulk."""
 
@@ -179,7 +186,7 @@ class MixedSolver:
             _lib.call("cc_basis_combine_f64", stream(), k, n, ptr(NU), ptr(coeffs), ptr(t))
             _lib.call("cc_axpby_f64", stream(), n, -rho, ptr(t), 1.0, ptr(r))
             _lib.call("cc_jd_correction_lp", stream(), self.Hlp.plan, f.n_env, f.d, ptr(f.Lt), ptr(f.W), ptr(f.Wt),
-                      ptr(small[0:1]), ptr(xr), ptr(r), int(inner_iters), ptr(self._jd_work), ptr(t))
+                      ptr(small[0:1]), ptr(xr), ptr(r), _schedule(inner_iters, outer), ptr(self._jd_work), ptr(t))
             # orthogonalise the correction against the basis (twice, Euclidean) and normalise
             for _ in range(2):
                 c = empty((k,))
@@ -236,7 +243,7 @@ class MixedSolver:
                 _lib.call("cc_dot_f64", stream(), n, ptr(tmp), ptr(tmp), ptr(hist[outer + 1:outer + 2]))
                 break
             _lib.call("cc_jd_correction_lp", stream(), self.Hlp.plan, f.n_env, f.d, ptr(f.Lt), ptr(f.W), ptr(f.Wt),
-                      ptr(small[0:1]), ptr(xr), ptr(r), int(inner_iters), ptr(self._jd_work), ptr(t))
+                      ptr(small[0:1]), ptr(xr), ptr(r), _schedule(inner_iters, k - 1), ptr(self._jd_work), ptr(t))
             for _ in range(2):
                 c = empty((k,))
                 _lib.call("cc_basis_dots_f64", stream(), k, n, ptr(U), ptr(t), ptr(c), 1)

@@ -104,7 +104,9 @@ int cc_svd_f64(void* stream, int64_t batch, int64_t m, int64_t n, const double* 
  * operator legs are fused into one index).  C is the corner's Identity channel matricized as
  * (x, a, b) x far: the fused bond (x = the corner's old bond, a / b = the absorbed side's center ket / bra
  * legs) against the corner's far bond (far <= 256, x a^2 >= far).  The rows are symmetrized under the
- * a <-> b exchange (so the projected environment keeps its ket <-> bra symmetry) and U_out (x a^2 by
+ * a <-> b exchange (so the projected environment keeps its ket <-> bra symmetry); the singular vectors come
+ * from the eigen-decomposition of the far x far Gram matrix (U = Cs V S^-1; columns whose singular value is
+ * at most 1e-7 of the largest are zero) and U_out (x a^2 by
  * new_dim, orthonormal columns, signs fixed by cc_sign_normalize_columns_f64) receives the leading new_dim
  * left singular vectors, S_out (far) all
  * singular values, descending.  new_dim <= far: the corner's rank bound, at which the Identity-channel

@@ -27,13 +27,15 @@ from .tags import Identity, L, R
 def fused_basis(Cm, nx, na, k):
     """Cm: (nx na na) x far, rows (x, a, b).  The k leading left singular vectors of its a <-> b
     symmetric part (each with the sign that makes its largest entry positive: the two ends of a side are
-    identified by the absorptions that follow, so the sign of a basis vector is not a free gauge), and
-    all singular values."""
+    identified by the absorptions that follow, so the sign of a basis vector is not a free gauge; columns
+    whose singular value is at most 1e-7 of the largest are zero), and all singular values."""
     far = Cm.shape[1]
     C3 = Cm.reshape(nx, na, na, far)
     Cs = 0.5 * (C3 + C3.transpose(0, 2, 1, 3))
     u, s, _ = np.linalg.svd(Cs.reshape(nx * na * na, far), full_matrices=False)
-    return linalg.sign_normalize_columns(u[:, :k]), s
+    u = linalg.sign_normalize_columns(u[:, :k])
+    u[:, s[:k] <= 1e-7 * s[0]] = 0.0      # directions below what a Gram-matrix eigensolve resolves
+    return u, s
 
 
 def _corner_basis(C, bond_legs, far_legs, new):
