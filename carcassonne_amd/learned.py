@@ -72,6 +72,11 @@ class WarmStartMLP:
     def _refresh16(self):
         _lib.call("cc_cast", stream(), self.n_params, _lib.CC_F32, ptr(self.P32), _lib.CC_F16, ptr(self.P16))
 
+    @classmethod
+    def for_residual(cls, n, n_out, hidden=512, seed=0):
+        """A network whose input is a residual row (residual_features: n values and h) instead of spectra."""
+        return cls(1, n, n_out, hidden=hidden, seed=seed)
+
     # -- features ---------------------------------------------------------------------------
     def features(self, corners, sides, h, out=None):
         """FP16 feature row (K0) of one environment."""
@@ -172,6 +177,102 @@ class Family:
         return corners, sides, opc, HipData.fromArray(st)
 
 
+def combined_start(H_operator, N_operator, vectors):
+    """The Rayleigh-Ritz combination of several start vectors (flat FP64 device tensors): the lowest
+    generalized eigenvector of the pencil (V^T H V, V^T N V) over their span, as one flat vector.  With the
+    previous step's state and the network's proposal it is never worse, in energy, than either alone; its
+    cost is one FP64 H and N product per vector (cc_small_geneig_f64 on the device)."""
+    k = len(vectors)
+    n = H_operator.n
+    V = torch.stack([v.reshape(-1) for v in vectors]).contiguous()
+    HV, NV = torch.empty_like(V), torch.empty_like(V)
+    for j in range(k):
+        H_operator.apply_flat(V[j], HV[j])
+        N_operator.apply_flat(V[j], NV[j])
+    hs, ns = empty((k, k)), empty((k, k))
+    for j in range(k):
+        _lib.call("cc_basis_dots_f64", stream(), k, n, ptr(V), ptr(HV[j]), ptr(hs[:, j]), k)
+        _lib.call("cc_basis_dots_f64", stream(), k, n, ptr(V), ptr(NV[j]), ptr(ns[:, j]), k)
+    small = empty((k + 1,))
+    _lib.call("cc_small_geneig_f64", stream(), k, ptr(hs), ptr(ns), ptr(small[0:1]), ptr(small[1:1 + k]))
+    coeffs = small[1:1 + k].contiguous()
+    out = empty((n,))
+    _lib.call("cc_basis_combine_f64", stream(), k, n, ptr(V), ptr(coeffs), ptr(out))
+    return out
+
+
+def residual_features(net, H_operator, N_operator, x_prev, h, out=None):
+    """FP16 input row of the correction network: the residual r = H x - rho N x of the previous step's state
+    x in the new environment (rho its Rayleigh quotient), scaled to unit norm, then h, zero-padded to K0.  The
+    residual carries what the new environment changes about the state (to first order the correction is
+    -(H - rho N)^-1 r off x), which the environment's spectra alone do not."""
+    n = H_operator.n
+    if net.K0 < n + 1:
+        raise ValueError("the network's input is narrower than the residual")
+    if out is None:
+        out = torch.zeros(net.K0, dtype=torch.float16, device=net.P32.device)
+    hx, nx, r = empty((n,)), empty((n,)), empty((n,))
+    H_operator.apply_flat(x_prev, hx)
+    N_operator.apply_flat(x_prev, nx)
+    d = empty((3,))
+    _lib.call("cc_dot_f64", stream(), n, ptr(x_prev), ptr(hx), ptr(d[0:1]))
+    _lib.call("cc_dot_f64", stream(), n, ptr(x_prev), ptr(nx), ptr(d[1:2]))
+    _lib.call("cc_binary_f64", stream(), 1, 1, ptr(d[0:1]), ptr(d[1:2]), ptr(d[2:3]))      # rho
+    _lib.call("cc_axpby_dev_f64", stream(), n, ptr(d[2:3]), -1.0, ptr(nx), 1.0, ptr(hx))    # hx -= rho nx
+    _lib.call("cc_normalize_f64", stream(), n, ptr(hx), ptr(r))
+    _lib.call("cc_cast", stream(), n, _lib.CC_F64, ptr(r), _lib.CC_F16, ptr(out[0:n]))
+    hv = torch.full((1,), float(h), dtype=torch.float64, device=out.device)
+    _lib.call("cc_cast", stream(), 1, _lib.CC_F64, ptr(hv), _lib.CC_F16, ptr(out[n:n + 1]))
+    return out
+
+
+def learned_correction_start(net, H_operator, N_operator, x_prev, h):
+    """The learned start of a sweep step: the network maps the previous state's residual in the new
+    environment to a correction direction y, and the start is the Rayleigh-Ritz combination of the previous
+    state and y (combined_start)."""
+    y = net.propose(residual_features(net, H_operator, N_operator, x_prev, h))[:H_operator.n].contiguous()
+    return combined_start(H_operator, N_operator, [x_prev, y])
+
+
+def generate_correction_pairs(family, net, n_chains, rng, h_start=(2.0, 2.3), h_step=(0.04, 0.13), seed0=1000,
+                              check_every=4):
+    """Training pairs of the correction network from walks that mimic consecutive sweep steps: each chain
+    starts at h ~ U[h_start] with a fresh perturbation seed and steps h by U[h_step] (new perturbation seed
+    every step) until h > 4.  For each step: the input row from the previous state's residual in the new
+    environment, and the target, the part of the new converged state orthogonal to the previous one (sign
+    aligned with it).  Returns (X16, T, number of pairs)."""
+    from .pipeline import multipliers
+    n = family.n
+    dev = net.P32.device
+    X, T = [], []
+    seed = seed0
+    for c in range(n_chains):
+        h = float(rng.uniform(*h_start))
+        corners, sides, opc, x0 = family.environment(seed, h)
+        seed += 1
+        x_prev, _, _ = solve_from(corners, sides, opc, x0, check_every=check_every)
+        while True:
+            h += float(rng.uniform(*h_step))
+            if h > 4.0:
+                break
+            corners, sides, opc, _ = family.environment(seed, h)
+            seed += 1
+            H, N = multipliers(corners, sides, opc)
+            X.append(residual_features(net, H.operator, N.operator, x_prev, h))
+            x, _, _ = solve_from(corners, sides, opc, x_prev, check_every=check_every)
+            d = empty((2,))
+            _lib.call("cc_dot_f64", stream(), n, ptr(x), ptr(x_prev), ptr(d[0:1]))
+            _lib.call("cc_dot_f64", stream(), n, ptr(x_prev), ptr(x_prev), ptr(d[1:2]))
+            _lib.call("cc_binary_f64", stream(), 1, 1, ptr(d[0:1]), ptr(d[1:2]), ptr(d[0:1]))
+            t = x.clone()
+            _lib.call("cc_axpby_dev_f64", stream(), n, ptr(d[0:1]), -1.0, ptr(x_prev), 1.0, ptr(t))   # t = x - c x_prev
+            tt = torch.zeros(net.NO, dtype=torch.float32, device=dev)
+            _lib.call("cc_cast", stream(), n, _lib.CC_F64, ptr(t), _lib.CC_F32, ptr(tt))
+            T.append(tt)
+            x_prev = x
+    return torch.stack(X).contiguous(), torch.stack(T).contiguous(), len(X)
+
+
 def solve_from(corners, sides, op_center, x0, check_every=1, tol_residual=1e-10):
     """Plain FP64 solve from start vector x0 (flat device tensor or HipData): (x, energy, iterations)."""
     from .pipeline import solve
@@ -196,14 +297,16 @@ def generate_pairs(family, model_net, sample_seeds, h_values, check_every=4):
     return X16, T, iters
 
 
-def evaluate_starts(family, model_net, sample_seeds, h_values, seed=0):
-    """Lanczos iterations to a 1e-10 residual on each held-out environment from three starts: a seeded
+def evaluate_starts(family, model_net, sample_seeds, h_values, seed=0, correction_net=None):
+    """Lanczos iterations to a 1e-10 residual on each held-out environment from four starts: a seeded
     random vector, the previous environment's converged state (the sequence is walked in order, as
-    consecutive sweep steps), and the network's proposal.  Energies of the three solves agree."""
+    consecutive sweep steps), the network's proposal, and the Rayleigh-Ritz combination of the last two
+    (combined_start; timed with its products).  Energies of the solves agree."""
     import time
     rng = np.random.Generator(np.random.PCG64(seed))
-    out = dict(random=[], previous=[], learned=[], energy_spread=[], ms_random=[], ms_previous=[], ms_learned=[],
-               overlap_learned=[], overlap_previous=[])
+    out = dict(random=[], previous=[], learned=[], combined=[], corrected=[], energy_spread=[], ms_random=[],
+               ms_previous=[], ms_learned=[], ms_combined=[], ms_corrected=[], overlap_learned=[],
+               overlap_previous=[], overlap_combined=[], overlap_corrected=[])
 
     def overlap(a, b):
         """|a . b|^2 / (|a|^2 |b|^2) on the device (cc_dot_f64)."""
@@ -244,6 +347,29 @@ def evaluate_starts(family, model_net, sample_seeds, h_values, seed=0):
             out["ms_previous"].append(ms)
             out["overlap_previous"].append(overlap(prev, x1))
             es.append(e3)
+
+            def combined():
+                from .pipeline import multipliers
+                H, N = multipliers(corners, sides, opc)
+                xl = model_net.propose(model_net.features(corners, sides, h))[:family.n].contiguous()
+                xc = combined_start(H.operator, N.operator, [prev, xl])
+                return solve_from(corners, sides, opc, xc) + (xc,)
+            (_, e4, it_c, xc), ms = timed(combined)
+            out["combined"].append(it_c)
+            out["ms_combined"].append(ms)
+            out["overlap_combined"].append(overlap(xc, x1))
+            es.append(e4)
+            if correction_net is not None:
+                def corrected():
+                    from .pipeline import multipliers
+                    H, N = multipliers(corners, sides, opc)
+                    xs = learned_correction_start(correction_net, H.operator, N.operator, prev, h)
+                    return solve_from(corners, sides, opc, xs) + (xs,)
+                (_, e5, it_x, xs), ms = timed(corrected)
+                out["corrected"].append(it_x)
+                out["ms_corrected"].append(ms)
+                out["overlap_corrected"].append(overlap(xs, x1))
+                es.append(e5)
         out["energy_spread"].append(max(es) - min(es))
         prev = x1
     return out

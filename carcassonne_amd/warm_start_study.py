@@ -31,6 +31,8 @@ def main(argv=None):
     ap.add_argument("--eval", type=int, default=40)
     ap.add_argument("--out", default=None)
     ap.add_argument("--save", default=None)
+    ap.add_argument("--chains", type=int, default=0,
+                    help="also train the correction network on this many walks of consecutive environments")
     args = ap.parse_args(argv)
 
     from .device import synchronize
@@ -58,16 +60,29 @@ def main(argv=None):
     synchronize()
     t_train = time.perf_counter() - t1
     print("trained: loss", ["%.3g" % h for h in hist], "%.1f s" % t_train, file=sys.stderr, flush=True)
+    cnet = None
+    if args.chains:
+        from .learned import generate_correction_pairs
+        t2 = time.perf_counter()
+        cnet = WarmStartMLP.for_residual(fam.n, _pad64(fam.n), hidden=args.hidden, seed=1)
+        Xc, Tc, npairs = generate_correction_pairs(fam, cnet, args.chains, np.random.default_rng(7), seed0=500000)
+        synchronize()
+        print("correction pairs", npairs, "%.1f s" % (time.perf_counter() - t2), file=sys.stderr, flush=True)
+        chist = cnet.train(Xc, Tc, epochs=args.epochs, batch=min(args.batch, npairs), lr=args.lr)
+        print("correction net: loss", ["%.3g" % h for h in chist], file=sys.stderr, flush=True)
     eval_seeds = list(range(900000, 900000 + args.eval))
-    ev = evaluate_starts(fam, net, eval_seeds, np.linspace(2.0, 4.0, args.eval))
+    ev = evaluate_starts(fam, net, eval_seeds, np.linspace(2.0, 4.0, args.eval), correction_net=cnet)
+    kinds = ("random", "previous", "learned", "combined") + (("corrected",) if cnet is not None else ())
     res = {
         "workload": "learned warm start, 2D TFIM family D={} chi={} sigma={} h~U[2,4]".format(args.D, args.chi, args.sigma),
         "pairs": args.pairs, "generation_s": t_gen, "train_s": t_train, "train_loss": hist,
-        "mean_iterations": {k: float(np.mean(ev[k])) for k in ("random", "previous", "learned")},
-        "solves_per_s": {k: 1e3 / float(np.mean(ev["ms_" + k])) for k in ("random", "previous", "learned")},
-        "mean_overlap_with_converged": {k: float(np.mean(ev["overlap_" + k])) for k in ("previous", "learned")},
+        # means over the environments that have a previous one (all but the first), for every start
+        "mean_iterations": {k: float(np.mean(ev[k][1:] if k in ("random", "learned") else ev[k])) for k in kinds},
+        "solves_per_s": {k: 1e3 / float(np.mean(ev["ms_" + k][1:] if k in ("random", "learned") else ev["ms_" + k]))
+                         for k in kinds},
+        "mean_overlap_with_converged": {k: float(np.mean(ev["overlap_" + k])) for k in kinds if k != "random"},
         "max_energy_spread": float(max(ev["energy_spread"])),
-        "iterations": {k: ev[k] for k in ("random", "previous", "learned")},
+        "iterations": {k: ev[k] for k in kinds},
     }
     line = json.dumps(res)
     print(line, flush=True)
