@@ -48,43 +48,85 @@ def golden_energy():
         return json.load(f)[key]["ground_energy"]
 
 
-def cpu_baseline(gpu_iterations, sample_iterations=6):
-    """The oracle (NumPy/SciPy FP64) on the same workload, timed on this host: stage 1/2,
-    normalization factor, and `sample_iterations` Lanczos steps; a solve is extrapolated to the
-    plain FP64 GPU form's iteration count."""
+def _oracle_operator(D, chi, seed, h=3.0):
+    from carcassonne_amd import synthetic as syn
+    from oracle import sparse2d
+    from tests.oracle_bridge import to_oracle
+    c, s, st = syn.make_environment(D, chi, model=WORKLOAD["model"], seed=seed)
+    return sparse2d.form_multipliers(*to_oracle(c, s, syn.operator_center(WORKLOAD["model"], h=h))), st
+
+
+def _median_time(fn, reps):
+    import numpy as np
+    fn()                                   # warm-up
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def cpu_baseline(gpu_iterations, sample_iterations=6, reps=5):
+    """The oracle (NumPy/SciPy FP64) timed on this host, as a bounded sample:
+
+    * headline size (D=6, chi=64): stage 1/2, the Cholesky factor of N_env and `sample_iterations` Lanczos
+      steps, the solve extrapolated to the plain FP64 GPU form's iteration count (all BLAS threads; a full
+      CPU solve there takes about half a minute);
+    * D=4, chi=32, h=3: complete solves -- the reference's relaxOver (restarted Arnoldi, k=3, 100
+      multiplications, its cost rule choosing dense H and LU of N) and the residual-controlled Lanczos
+      port to 1e-10 -- one warm-up, then the median of `reps` repetitions, with one BLAS thread and with
+      all of them."""
     import numpy as np
     from scipy.linalg import cholesky, solve_triangular
-    from threadpoolctl import threadpool_info
-    from carcassonne_amd import synthetic as syn
-    from oracle import dense, sparse2d
-    from tests.oracle_bridge import to_oracle
-    c, s, st = syn.make_environment(WORKLOAD["D"], WORKLOAD["chi"], model=WORKLOAD["model"], seed=WORKLOAD["seed"])
-    t0 = time.perf_counter()
-    op = sparse2d.form_multipliers(*to_oracle(c, s, syn.operator_center(WORKLOAD["model"], h=WORKLOAD["h"])))
-    Lc = cholesky(dense.normalization_env_matrix(op.AN, op.BN), lower=True)
-    t1 = time.perf_counter()
-    d = op.d
-    ne = Lc.shape[0]
-    q = (Lc.T @ st.reshape(ne, d)).ravel()
-    q /= np.linalg.norm(q)
-    Q = [q]
-    t2 = time.perf_counter()
-    for _ in range(sample_iterations):
-        z = solve_triangular(Lc, Q[-1].reshape(ne, d), lower=True, trans='T')
-        w = solve_triangular(Lc, op.apply_H(z).reshape(ne, d), lower=True).ravel()
-        Qm = np.array(Q)
-        w -= Qm.T @ (Qm @ w)
-        w -= Qm.T @ (Qm @ w)
-        Q.append(w / np.linalg.norm(w))
-    t3 = time.perf_counter()
-    per_iter = (t3 - t2) / sample_iterations
-    t_solve = (t1 - t0) + gpu_iterations * per_iter
+    from threadpoolctl import threadpool_info, threadpool_limits
+    from oracle import dense, solver as OS
     threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
+
+    def headline_sample():
+        t0 = time.perf_counter()
+        op, st = _oracle_operator(WORKLOAD["D"], WORKLOAD["chi"], WORKLOAD["seed"], WORKLOAD["h"])
+        Lc = cholesky(dense.normalization_env_matrix(op.AN, op.BN), lower=True)
+        t1 = time.perf_counter()
+        d, ne = op.d, Lc.shape[0]
+        q = (Lc.T @ st.reshape(ne, d)).ravel()
+        Q = [q / np.linalg.norm(q)]
+        t2 = time.perf_counter()
+        for _ in range(sample_iterations):
+            z = solve_triangular(Lc, Q[-1].reshape(ne, d), lower=True, trans='T')
+            w = solve_triangular(Lc, op.apply_H(z).reshape(ne, d), lower=True).ravel()
+            Qm = np.array(Q)
+            w -= Qm.T @ (Qm @ w)
+            w -= Qm.T @ (Qm @ w)
+            Q.append(w / np.linalg.norm(w))
+        t3 = time.perf_counter()
+        return (t1 - t0) + gpu_iterations * (t3 - t2) / sample_iterations, t3 - t0
+
+    t_solve, t_sample = headline_sample()
+    op4, st4 = _oracle_operator(4, 32, 0)
+    H4, N4 = op4.matrix_H(), op4.matrix_N()
+    use_dense = OS.is_cheaper_to_form_matrix(6, op4.cost_H, op4.cost_H_form, op4.n)
+
+    def relax():
+        OS.relax_over(st4.ravel(), lambda x: op4.apply_H(x).ravel(), lambda x: op4.apply_N(x).ravel(), max_mults=100,
+                      tol=1e-7, H_matrix=op4.matrix_H() if use_dense else None, N_matrix=op4.matrix_N(),
+                      use_dense_H=use_dense, use_lu_N=True)
+
+    def lanczos():
+        OS.lanczos_generalized(lambda x: op4.apply_H(x).ravel(), op4.matrix_N(), st4.ravel(), tol=1e-10)
+
+    d4 = {}
+    for name, fn in (("relaxover", relax), ("lanczos", lanczos)):
+        with threadpool_limits(1):
+            d4[name + "_1_thread_s"] = _median_time(fn, reps)
+        d4[name + "_all_threads_s"] = _median_time(fn, reps)
     return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": int(threads), "kind": "port",
-            "sample": "oracle stage 1/2 + Cholesky of N_env + {} Lanczos steps at D=6, chi=64 ({:.1f} s); solve "
-                      "time extrapolated to the plain FP64 form's {} iterations".format(sample_iterations, t3 - t0,
-                                                                                        gpu_iterations),
-            "host_cpus": len(os.sched_getaffinity(0))}
+            "sample": "oracle stage 1/2 + Cholesky of N_env + {} Lanczos steps at D=6, chi=64 ({:.1f} s); solve time "
+                      "extrapolated to the plain FP64 form's {} iterations".format(sample_iterations, t_sample,
+                                                                                 gpu_iterations),
+            "host_cpus": len(os.sched_getaffinity(0)),
+            "d4_chi32_full_solves": dict(d4, reps=reps, statistic="median after one warm-up",
+                                         solves_per_s={k[:-2]: 1.0 / v for k, v in d4.items()})}
 
 
 def main():
@@ -202,6 +244,10 @@ def main():
                                "factor + eigensolve to 1e-10 rel. energy + FP64 Rayleigh quotient",
                    "form": form, "D": 6, "chi": 64},
         "energy": head["energy"], "energy_rel_err_vs_oracle": head["err"],
+        "learned_warm_start": "not used by the headline form: at the learned-start workload (D=4, chi=32) the trained "
+                              "network's start takes 50 Lanczos iterations against 46 from the previous-step start "
+                              "and 67 from a random one (profiles/warm_start_D4_chi32_study.md); both forms here "
+                              "start from the same seeded state",
     }
     if "fp64" in out:
         f = out["fp64"]
