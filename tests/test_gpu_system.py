@@ -297,3 +297,77 @@ def test_full_eigensolver_matches_oracle_eigh(gpu, walk, axes):
         s.minimizeExpectationUsingFullEigensolver()
     e_p, e_o = psys.computeExpectation(), osys.computeExpectation()
     assert abs(e_p - e_o) <= 1e-12 * max(1.0, abs(e_o)), (e_p, e_o)
+
+
+def test_gauge_moves_match_oracle(gpu):
+    """normalizeCornerAndDenormalizeSide / normalizeSideAndDenormalizeCenter / normalizeCenterAndDenormalizeSide
+    and the full normalize() (system/_2d.py:503-549) on the GPU and in the oracle from the same tensors.
+    The normalizer is diag(S^-1/2) V* of the SVD M = U S V (data/__init__.py:263-301), so each move is fixed up
+    to the sign of every right singular vector; a sign flips index a of the ket leg and of the bra leg
+    together in the normalized tensor and in its neighbour, which leaves every entry's magnitude unchanged.
+    The magnitudes of every corner, side and center entry are compared, besides the invariant energy and
+    normalization."""
+    from tests.oracle_bridge import product_to_oracle_tag, rel_err
+
+    def compare_systems(psys, osys):
+        worst = 0.0
+        for pl, ol in ((psys.corners, osys.corners), (psys.sides, osys.sides)):
+            for pd, od in zip(pl, ol):
+                for t, v in pd.items():
+                    worst = max(worst, rel_err(np.abs(v.toArray()), np.abs(od[product_to_oracle_tag(t)])))
+        return max(worst, rel_err(np.abs(psys.state_center_data.toArray()), np.abs(osys.state)))
+
+    psys, osys = _pair(5, maximum_dimension=3)
+    for s in (psys, osys):
+        s.contractTowards(0)
+        s.contractTowards(1)
+    for corner_id, direction in ((2, 1), (0, 0), (3, 1)):
+        psys.normalizeCornerAndDenormalizeSide(corner_id, direction)
+        osys.normalizeCornerAndDenormalizeSide(corner_id, direction)
+        assert compare_systems(psys, osys) < 1e-9, ("corner", corner_id, direction)
+    for side_id in (3, 1):
+        psys.normalizeSideAndDenormalizeCenter(side_id)
+        osys.normalizeSideAndDenormalizeCenter(side_id)
+        assert compare_systems(psys, osys) < 1e-9, ("side", side_id)
+    psys.normalizeCenterAndDenormalizeSide(0)
+    osys.normalizeCenterAndDenormalizeSide(0)
+    assert compare_systems(psys, osys) < 1e-9
+    psys.normalize()
+    osys.normalize()
+    assert compare_systems(psys, osys) < 1e-8
+    e_p, n_p = psys.computeExpectationAndNormalization()
+    e_o, n_o = osys.computeExpectationAndNormalization()
+    assert abs(e_p - e_o) <= 1e-9 * abs(e_o) and abs(n_p - n_o) <= 1e-9 * abs(n_o)
+
+
+@pytest.mark.parametrize("seed,new", [(8, 1), (9, 2), (12, 2)])
+def test_state_compression_matches_oracle(gpu, seed, new):
+    """compressCornerStateTowards (system/_2d.py:191-228, compression.py:26-45) with the deterministic start on
+    both sides: the compressors agree with the oracle's, and so do the compressed corners and sides, the
+    gauge-invariant product of each compressed corner with its side over the bond (L P R), the energy and
+    the normalization."""
+    import numpy as np
+    from carcassonne_amd.sparse import Identity
+    from oracle.tags import Identity as OIdentity
+    from tests.oracle_bridge import compare_systems
+    psys, osys = _pair(seed, maximum_dimension=2)
+    for s in (psys, osys):
+        s.compressor_start = "svd"
+        for d in (0, 1, 2, 3):
+            s.contractTowards(d)
+    for corner_id in range(4):
+        for direction in range(2):
+            dim = osys.corners[corner_id][OIdentity()].shape[3 * direction]
+            target = max(1, min(new, dim - 1))
+            psys.compressCornerStateTowards(corner_id, direction, target)
+            osys.compressCornerStateTowards(corner_id, direction, target)
+    assert compare_systems(psys, osys) < 1e-7
+    for corner_id in range(4):
+        pc, oc = psys.corners[corner_id][Identity()].toArray(), osys.corners[corner_id][OIdentity()]
+        ps, os_ = psys.sides[corner_id][Identity()].toArray(), osys.sides[corner_id][OIdentity()]
+        lpr_p = np.tensordot(pc, ps, ((3, 4, 5), (0, 1, 2)))
+        lpr_o = np.tensordot(oc, os_, ((3, 4, 5), (0, 1, 2)))
+        assert np.linalg.norm(lpr_p - lpr_o) <= 1e-7 * np.linalg.norm(lpr_o)
+    e_p, n_p = psys.computeExpectationAndNormalization()
+    e_o, n_o = osys.computeExpectationAndNormalization()
+    assert abs(e_p - e_o) <= 1e-8 * abs(e_o) and abs(n_p - n_o) <= 1e-8 * abs(n_o)
