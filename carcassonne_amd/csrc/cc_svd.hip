@@ -1,6 +1,8 @@
 // Thin SVD by one-sided Jacobi (Hestenes) with the round-robin parallel ordering:
 // one 1024-thread workgroup per matrix, one wave per disjoint column pair per round.
-// Columns live in global memory (`work`), so m is unbounded; n <= 256.
+// The columns of A and V live in LDS when (m + n) n <= 8192 doubles (64 KiB, every Gram matrix of a
+// bond up to 64 and the small normalizations), otherwise in global memory (`work`), so m is unbounded;
+// n <= 256.  Both placements do the same arithmetic in the same order.
 #include "cc_common.h"
 
 namespace {
@@ -14,6 +16,17 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+constexpr int SVD_LDS_DOUBLES = 8192;
+
+// Player at position e (0 <= e < np) of round r of the round-robin tournament: position 0 keeps player 0,
+// the others rotate by one position per round (after np - 1 rounds every pair has met once).
+__device__ inline int tournament_player(int e, int r, int np) {
+  if (e == 0) return 0;
+  const int k = np - 1;
+  return 1 + ((e - 1 - r) % k + k) % k;
+}
+
+template <bool IN_LDS>
 __global__ void __launch_bounds__(SVD_THREADS) svd_jacobi_kernel(int64_t m, int n, const double* __restrict__ A,
                                                                  double* __restrict__ U, double* __restrict__ S,
                                                                  double* __restrict__ Vt, double* __restrict__ work) {
@@ -22,10 +35,10 @@ __global__ void __launch_bounds__(SVD_THREADS) svd_jacobi_kernel(int64_t m, int 
   U += b * m * n;
   S += b * n;
   Vt += b * n * n;
-  double* W = work + b * (m * n + (int64_t)n * n);  // W[j][i] = column j of A (n x m)
-  double* V = W + m * n;                             // V[j][k] = column j of V (n x n)
+  __shared__ double cols[IN_LDS ? SVD_LDS_DOUBLES : 1];
+  double* W = IN_LDS ? cols : work + b * (m * n + (int64_t)n * n);  // W[j][i] = column j of A (n x m)
+  double* V = W + m * n;                                             // V[j][k] = column j of V (n x n)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = SVD_THREADS / 64;
-  __shared__ int pairs[SVD_MAX_N + 1];
   __shared__ int rotated;
   __shared__ double snorm[SVD_MAX_N];
   __shared__ int perm[SVD_MAX_N];
@@ -37,7 +50,6 @@ __global__ void __launch_bounds__(SVD_THREADS) svd_jacobi_kernel(int64_t m, int 
   }
   for (int e = tid; e < n * n; e += SVD_THREADS) V[e] = (e / n == e % n) ? 1.0 : 0.0;
   const int np = (n + 1) & ~1;  // even count; index n (when odd) is a dummy
-  for (int e = tid; e < np; e += SVD_THREADS) pairs[e] = e;
   __syncthreads();
 
   for (int sweep = 0; sweep < 60; ++sweep) {
@@ -45,7 +57,7 @@ __global__ void __launch_bounds__(SVD_THREADS) svd_jacobi_kernel(int64_t m, int 
     __syncthreads();
     for (int round = 0; round < np - 1; ++round) {
       for (int pr = wave; pr < np / 2; pr += nwaves) {
-        int p = pairs[pr], q = pairs[np - 1 - pr];
+        int p = tournament_player(pr, round, np), q = tournament_player(np - 1 - pr, round, np);
         if (p >= n || q >= n) continue;
         if (p > q) { int t = p; p = q; q = t; }
         double* wp = W + (int64_t)p * m;
@@ -78,13 +90,6 @@ __global__ void __launch_bounds__(SVD_THREADS) svd_jacobi_kernel(int64_t m, int 
           }
           if (lane == 0) rotated = 1;
         }
-      }
-      __syncthreads();
-      // rotate the tournament: keep pairs[0], shift the rest by one
-      if (tid == 0) {
-        int last = pairs[np - 1];
-        for (int e = np - 1; e > 1; --e) pairs[e] = pairs[e - 1];
-        pairs[1] = last;
       }
       __syncthreads();
     }
@@ -201,6 +206,153 @@ __global__ void __launch_bounds__(SVD_THREADS) svd_jacobi_kernel(int64_t m, int 
     __syncthreads();
   }
 }
+// Eigen-decomposition of a symmetric n x n matrix (n <= 64) by the parallel cyclic Jacobi method, matrix and
+// eigenvectors in LDS: each round of the round-robin ordering rotates n/2 disjoint (p, q) pairs, first the
+// columns of G and V (G J, V J) and then the rows of G (J^T G J).  A pair is rotated while
+// |g_pq| > 1e-15 sqrt(|g_pp g_qq|) and |g_pq| > 1e-15 tr|G| (off-diagonal entries below the second bound move
+// the eigenvalues by less than the rounding of G itself, and rotating them would not terminate for
+// rank-deficient G).  Outputs: lam descending (stable by index) and Vt[r][i] = component i of eigenvector r.
+constexpr int SYM_THREADS = 1024;
+constexpr int SYM_MAX_N = 64;
+constexpr int SYM_TASKS = SYM_MAX_N * (SYM_MAX_N / 2) / SYM_THREADS;  // (entry pair) tasks per thread and phase
+static_assert(SYM_TASKS * SYM_THREADS == SYM_MAX_N * (SYM_MAX_N / 2), "tasks must cover n * n/2 exactly");
+
+template <int NC>  // NC = n known at compile time (index arithmetic by shifts and constant divisors), 0 = any n
+__global__ void __launch_bounds__(SYM_THREADS) sym_jacobi_kernel(int n_rt, const double* __restrict__ A,
+                                                                 double* __restrict__ lam, double* __restrict__ Vt) {
+  const int n = NC ? NC : n_rt;
+  __shared__ double G[SYM_MAX_N * SYM_MAX_N];
+  __shared__ double V[SYM_MAX_N * SYM_MAX_N];
+  __shared__ double rot_c[SYM_MAX_N / 2], rot_s[SYM_MAX_N / 2];
+  __shared__ int rot_p[SYM_MAX_N / 2], rot_q[SYM_MAX_N / 2], rot_on[SYM_MAX_N / 2];
+  __shared__ int perm[SYM_MAX_N];
+  __shared__ double trace_abs;
+  __shared__ int rotated;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < n * n; e += SYM_THREADS) {
+    G[e] = A[e];
+    V[e] = (e / n == e % n) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double t = 0.0;
+    for (int i = 0; i < n; ++i) t += fabs(G[i * n + i]);
+    trace_abs = t;
+  }
+  __syncthreads();
+  const double floor_abs = 1e-15 * trace_abs;
+  const int np = (n + 1) & ~1, half = np / 2;
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    if (tid == 0) rotated = 0;
+    __syncthreads();
+    for (int round = 0; round < np - 1; ++round) {
+      if (tid < half) {
+        int p = tournament_player(tid, round, np), q = tournament_player(np - 1 - tid, round, np);
+        if (p > q) { const int t = p; p = q; q = t; }
+        int on = 0;
+        if (q < n) {
+          const double a = G[p * n + p], b = G[q * n + q], g = G[p * n + q];
+          if (fabs(g) > 1e-15 * sqrt(fabs(a * b)) && fabs(g) > floor_abs) {
+            const double theta = (b - a) / (2.0 * g);
+            const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+            const double c = 1.0 / sqrt(t * t + 1.0);
+            rot_c[tid] = c;
+            rot_s[tid] = t * c;
+            on = 1;
+            rotated = 1;
+          }
+        }
+        rot_p[tid] = p;
+        rot_q[tid] = q;
+        rot_on[tid] = on;
+      }
+      __syncthreads();
+      // The tasks of a thread touch disjoint entries, so all of its loads are issued before its stores.
+      {  // columns p, q of G and V
+        int ip[SYM_TASKS], iq[SYM_TASKS];
+        double c[SYM_TASKS], sn[SYM_TASKS], gx[SYM_TASKS], gy[SYM_TASKS], vx[SYM_TASKS], vy[SYM_TASKS];
+        bool act[SYM_TASKS];
+#pragma unroll
+        for (int it = 0; it < SYM_TASKS; ++it) {
+          const int e = tid + it * SYM_THREADS, k = e % half, i = e / half;
+          act[it] = e < n * half && rot_on[k];
+          if (act[it]) {
+            ip[it] = i * n + rot_p[k];
+            iq[it] = i * n + rot_q[k];
+            c[it] = rot_c[k];
+            sn[it] = rot_s[k];
+            gx[it] = G[ip[it]];
+            gy[it] = G[iq[it]];
+            vx[it] = V[ip[it]];
+            vy[it] = V[iq[it]];
+          }
+        }
+#pragma unroll
+        for (int it = 0; it < SYM_TASKS; ++it) {
+          if (act[it]) {
+            G[ip[it]] = c[it] * gx[it] - sn[it] * gy[it];
+            G[iq[it]] = sn[it] * gx[it] + c[it] * gy[it];
+            V[ip[it]] = c[it] * vx[it] - sn[it] * vy[it];
+            V[iq[it]] = sn[it] * vx[it] + c[it] * vy[it];
+          }
+        }
+      }
+      __syncthreads();
+      {  // rows p, q of G
+        int ip[SYM_TASKS], iq[SYM_TASKS];
+        double c[SYM_TASKS], sn[SYM_TASKS], gx[SYM_TASKS], gy[SYM_TASKS];
+        bool act[SYM_TASKS];
+#pragma unroll
+        for (int it = 0; it < SYM_TASKS; ++it) {
+          const int e = tid + it * SYM_THREADS, j = e % n, k = e / n;
+          act[it] = e < n * half && rot_on[k];
+          if (act[it]) {
+            ip[it] = rot_p[k] * n + j;
+            iq[it] = rot_q[k] * n + j;
+            c[it] = rot_c[k];
+            sn[it] = rot_s[k];
+            gx[it] = G[ip[it]];
+            gy[it] = G[iq[it]];
+          }
+        }
+#pragma unroll
+        for (int it = 0; it < SYM_TASKS; ++it) {
+          if (act[it]) {
+            G[ip[it]] = c[it] * gx[it] - sn[it] * gy[it];
+            G[iq[it]] = sn[it] * gx[it] + c[it] * gy[it];
+          }
+        }
+      }
+      __syncthreads();
+    }
+    const int any = rotated;
+    __syncthreads();
+    if (!any) break;
+  }
+  for (int j = tid; j < n; j += SYM_THREADS) {
+    const double dj = G[j * n + j];
+    int r = 0;
+    for (int k = 0; k < n; ++k) {
+      const double dk = G[k * n + k];
+      r += (dk > dj) || (dk == dj && k < j);
+    }
+    perm[r] = j;
+  }
+  __syncthreads();
+  for (int r = tid; r < n; r += SYM_THREADS) lam[r] = G[perm[r] * n + perm[r]];
+  for (int e = tid; e < n * n; e += SYM_THREADS) {
+    const int r = e / n, i = e % n;
+    Vt[e] = V[i * n + perm[r]];
+  }
+}
+
+void launch_svd(hipStream_t st, unsigned batch, int64_t m, int n, const double* A, double* U, double* S, double* Vt,
+                double* work) {
+  if ((m + n) * n <= SVD_LDS_DOUBLES)
+    svd_jacobi_kernel<true><<<batch, SVD_THREADS, 0, st>>>(m, n, A, U, S, Vt, work);
+  else
+    svd_jacobi_kernel<false><<<batch, SVD_THREADS, 0, st>>>(m, n, A, U, S, Vt, work);
+}
 }  // namespace
 
 extern "C" int cc_svd_work_size(int64_t batch, int64_t m, int64_t n, int64_t* out) {
@@ -214,7 +366,7 @@ extern "C" int cc_svd_f64(void* stream, int64_t batch, int64_t m, int64_t n, con
   if (n > SVD_MAX_N) { cc_set_last_error("cc_svd_f64: more than 256 columns"); return CC_ERR_UNSUPPORTED; }
   if (m < n) { cc_set_last_error("cc_svd_f64: needs m >= n"); return CC_ERR_ARG; }
   if (batch == 0 || n == 0) return CC_OK;
-  svd_jacobi_kernel<<<(unsigned)batch, SVD_THREADS, 0, cc_stream(stream)>>>(m, (int)n, A, U, S, Vt, work);
+  launch_svd(cc_stream(stream), (unsigned)batch, m, (int)n, A, U, S, Vt, work);
   CC_LAUNCH_CHECK();
   return CC_OK;
 }
@@ -296,7 +448,8 @@ extern "C" int cc_fused_truncation_work_size(int64_t nx, int64_t na, int64_t far
 }
 
 // The leading left singular vectors of the symmetrized C (m x far, m >= far) from its Gram matrix:
-// G = Cs^T Cs = V diag(lambda) V^T (one-sided Jacobi on the far x far G), U = Cs V diag(1/sqrt(lambda)).
+// G = Cs^T Cs = V diag(lambda) V^T (two-sided Jacobi on the far x far G for far <= 64, else one-sided Jacobi),
+// U = Cs V diag(1/sqrt(lambda)).
 extern "C" int cc_fused_truncation_f64(void* stream, int64_t nx, int64_t na, int64_t far, int64_t new_dim,
                                        const double* C, double* U_out, double* S_out, double* work) {
   const int64_t m = nx * na * na;
@@ -318,7 +471,12 @@ extern "C" int cc_fused_truncation_f64(void* stream, int64_t nx, int64_t na, int
   CC_LAUNCH_CHECK();
   int rc = cc_gemm_f64(stream, 1, far, far, m, 1.0, Cs, 0, 1, far, Cs, 0, far, 1, 0.0, G, 0, far, 1);
   if (rc) return rc;
-  svd_jacobi_kernel<<<1, SVD_THREADS, 0, st>>>(far, (int)far, G, Ug, lam, Vt, work);
+  if (far == SYM_MAX_N)
+    sym_jacobi_kernel<SYM_MAX_N><<<1, SYM_THREADS, 0, st>>>((int)far, G, lam, Vt);
+  else if (far < SYM_MAX_N)
+    sym_jacobi_kernel<0><<<1, SYM_THREADS, 0, st>>>((int)far, G, lam, Vt);
+  else
+    launch_svd(st, 1, far, (int)far, G, Ug, lam, Vt, work);
   CC_LAUNCH_CHECK();
   fused_gram_scale_kernel<<<1, 256, 0, st>>>(far, new_dim, lam, Vt, W, S_out);
   CC_LAUNCH_CHECK();

@@ -93,3 +93,34 @@ def test_plain_cpu_workload_sweep_steps_match_oracle(gpu):
     assert all(close[:6]) and sum(close) >= len(close) // 3, close
     # a loose physical check of the trajectory (the one-site estimates fluctuate from step to step)
     assert -3.4 < float(np.mean(per_site[-8:])) < -2.8, per_site
+
+
+@pytest.mark.parametrize("nx,D,far,k,rank", [(32, 3, 64, 40, 48), (8, 2, 16, 16, 16), (6, 4, 50, 7, 50)])
+def test_fused_truncation_basis_matches_oracle(gpu, nx, D, far, k, rank):
+    """cc_fused_truncation_f64 (through system._2d.fusedTruncationBasis) against the oracle's SVD of the
+    symmetrized bond matricization (oracle/fused.py fused_basis) on a ket/bra-symmetric matrix with a graded
+    spectrum (1 down to 1e-4) and a null space: the columns whose singular value is at least 1e-2 of the
+    largest agree to 1e-9 (signs fixed by the same rule), and every kept column is a unit vector orthogonal
+    to the others."""
+    from carcassonne_amd.data import HipData
+    from carcassonne_amd.system._2d import fusedTruncationBasis
+    from oracle import fused
+    rng = np.random.default_rng(nx * 1000 + far)
+    m = nx * D * D
+    C3 = rng.standard_normal((nx, D, D, far))
+    Cs = (0.5 * (C3 + C3.transpose(0, 2, 1, 3))).reshape(m, far)
+    u, _, vt = np.linalg.svd(Cs, full_matrices=False)
+    r = min(rank, u.shape[1])
+    s = np.zeros(far)
+    s[:r] = np.logspace(0, -4, r)
+    Cm = (u[:, :far] * s[:u.shape[1]]) @ vt
+    corner = HipData.fromArray(Cm.T.reshape(far, 1, 1, nx * D, D, 1))
+    U = fusedTruncationBasis(corner, (3, 4, 5), (0, 1, 2), k).toArray()
+    want, sv = fused.fused_basis(Cm, nx, D, min(k, far))
+    assert U.shape == want.shape
+    big = sv[:U.shape[1]] >= 1e-2 * sv[0]
+    np.testing.assert_allclose(U[:, big], want[:, big], atol=1e-9)
+    kept = sv[:U.shape[1]] > 1e-7 * sv[0]
+    G = U[:, kept].T @ U[:, kept]
+    assert np.abs(G - np.eye(G.shape[0])).max() < 1e-6, np.abs(G - np.eye(G.shape[0])).max()
+    assert not np.any(U[:, ~kept])
