@@ -138,6 +138,26 @@ __device__ inline typename LPType<T>::vec8 lds8(const T* s, int row, int t, int 
   return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7);
 }
 
+// Stage A with the vector's hi and lo parts stacked along K: stacked index k < P holds hi at p = k,
+// P <= k < 2P holds lo at p = k - P, and the vector fragments are zero past 2P.  That takes ceil(2P/32) K
+// tiles instead of two products on each of ceil(P/32) tiles (3 instead of 4 at P = 36).  It needs
+// P % 4 == 0, so that no aligned 4-element run of K crosses P or 2P.
+template <int P, int SPLIT>
+__host__ __device__ constexpr bool lp_stacked() { return SPLIT != LP_SPLIT_NONE && P % 4 == 0; }
+template <int P, int SPLIT>
+__host__ __device__ constexpr int lp_ka() { return lp_stacked<P, SPLIT>() ? (2 * P + 31) / 32 : (P + 31) / 32; }
+// Slab column read for stacked index k: k mod P below 2P; past 2P (zero fragments) any finite entry, here
+// k - 2P inside the row, or k itself, which stays below 32 ceil(P/32) (covered by lp_pad) whenever
+// k - 2P >= P
+template <int P>
+__device__ inline int lp_scol(int k) { return k < P ? k : (k < 2 * P ? k - P : (k - 2 * P < P ? k - 2 * P : k)); }
+template <typename T, int P>
+__device__ inline typename LPType<T>::vec8 lds8s(const T* s, int row, int t, int g) {
+  const typename LPType<T>::vec4 a = lds4<T, P>(s, row, lp_scol<P>(32 * t + 4 * g)),
+                                 b = lds4<T, P>(s, row, lp_scol<P>(32 * t + 16 + 4 * g));
+  return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+
 // Direct global->LDS staging of one alpha's chunk slabs: wave w takes slots w, w + NT, ...; each
 // 64-lane instruction moves 64 consecutive 16-byte pieces of one slab (slot base pointers are
 // wave-uniform scalar loads, so no vector load sits between the issues).
@@ -258,13 +278,15 @@ __device__ __forceinline__ void lp_term_d(f32x4 (&y)[(P + 15) / 16][DP], f32x4 (
 // zeros with zero coefficients.
 template <int P, int DP, typename T, int SPLIT, bool DIAGF>
 __device__ __attribute__((always_inline)) inline void lp_alpha(const LPMeta<DP>& meta, int n_runs, const T* __restrict__ lds,
-                                const typename LPType<T>::vec8 (&vt)[DP * ((P + 15) / 16)][(P + 31) / 32],
+                                const typename LPType<T>::vec8 (&vt)[DP * ((P + 15) / 16)][lp_ka<P, SPLIT>()],
                                 const typename LPType<T>::vec8 (&vl)[DP * ((P + 15) / 16)][(P + 31) / 32],
                                 f32x4 (&y)[(P + 15) / 16][DP], f32x4 (&zp)[(P + 15) / 16][DP], float (&cp)[DP * DP],
                                 float (&cpd)[DP], int np, int lane) {
   typedef LPType<T> LT;
   typedef typename LT::vec8 V;
   constexpr int NT = (P + 15) / 16, QT = NT, KT = (P + 31) / 32, MT = DP * QT, SP = lp_stride(P);
+  constexpr bool STACK = lp_stacked<P, SPLIT>();
+  constexpr int KA = lp_ka<P, SPLIT>();
   const int g = lane >> 4;
   // B operands of the next term, loaded during the current term's products (the chunk's terms run in
   // order across its runs; the last term reloads itself)
@@ -278,11 +300,12 @@ __device__ __attribute__((always_inline)) inline void lp_alpha(const LPMeta<DP>&
       for (int nq = 0; nq < NT; ++nq) bop[t][nq] = lds8<T, P>(Bs, 16 * nq + (lane & 15), t, g);
   }
   // A operands of the next run, likewise loaded during the current run's terms
-  V aop[KT];
+  V aop[KA];
   {
     const T* As = lds + lp_lane_int(meta.rslot, 0) * SP;
 #pragma unroll
-    for (int t = 0; t < KT; ++t) aop[t] = lds8<T, P>(As, 16 * np + (lane & 15), t, g);
+    for (int t = 0; t < KA; ++t)
+      aop[t] = STACK ? lds8s<T, P>(As, 16 * np + (lane & 15), t, g) : lds8<T, P>(As, 16 * np + (lane & 15), t, g);
   }
   for (int r = 0; r < n_runs; ++r) {
     const int t0 = lp_lane_int(meta.rt0, r), t1 = lp_lane_int(meta.rt0, r + 1);
@@ -291,20 +314,28 @@ __device__ __attribute__((always_inline)) inline void lp_alpha(const LPMeta<DP>&
       f32x4 w[MT];
 #pragma unroll
       for (int m = 0; m < MT; ++m) w[m] = (f32x4){0.f, 0.f, 0.f, 0.f};
+      if constexpr (STACK) {
 #pragma unroll
-      for (int t = 0; t < KT; ++t) {
-        // hi products of all MT tiles, then the lo products: dependent MFMAs are MT issues apart
+        for (int t = 0; t < KA; ++t)
 #pragma unroll
-        for (int m = 0; m < MT; ++m) w[m] = LT::mfma(vt[m][t], aop[t], w[m]);
-        if (SPLIT != LP_SPLIT_NONE) {
+          for (int m = 0; m < MT; ++m) w[m] = LT::mfma(vt[m][t], aop[t], w[m]);
+      } else {
 #pragma unroll
-          for (int m = 0; m < MT; ++m) w[m] = LT::mfma(vl[m][t], aop[t], w[m]);
+        for (int t = 0; t < KT; ++t) {
+          // hi products of all MT tiles, then the lo products: dependent MFMAs are MT issues apart
+#pragma unroll
+          for (int m = 0; m < MT; ++m) w[m] = LT::mfma(vt[m][t], aop[t], w[m]);
+          if (SPLIT != LP_SPLIT_NONE) {
+#pragma unroll
+            for (int m = 0; m < MT; ++m) w[m] = LT::mfma(vl[m][t], aop[t], w[m]);
+          }
         }
       }
       {
         const T* As = lds + lp_lane_int(meta.rslot, r + 1 < n_runs ? r + 1 : r) * SP;
 #pragma unroll
-        for (int t = 0; t < KT; ++t) aop[t] = lds8<T, P>(As, 16 * np + (lane & 15), t, g);
+        for (int t = 0; t < KA; ++t)
+          aop[t] = STACK ? lds8s<T, P>(As, 16 * np + (lane & 15), t, g) : lds8<T, P>(As, 16 * np + (lane & 15), t, g);
       }
 #pragma unroll
       for (int s = 0; s < DP; ++s)
@@ -390,8 +421,10 @@ __global__ void __launch_bounds__(64 * ((P + 15) / 16), 2) lp_kernel(const LPDev
   constexpr int NT = (P + 15) / 16, KT = (P + 31) / 32, MT = DP * NT, NY = P * P * DP;
   constexpr int CPS = lp_stride(P) * 2 / 16, BUF = lp_slots(P) * CPS + lp_pad(P);
   constexpr int NTH = 64 * NT, NV = (NY + NTH - 1) / NTH;
-  // stage-A operand fragments in LDS, vec8 [hi | lo][m][t][lane]
-  constexpr int FT = MT * KT * 64, NF = SPLIT != LP_SPLIT_NONE ? 2 : 1;
+  // stage-A operand fragments in LDS, vec8 [hi | lo][m][t][lane] (one stacked set when lp_stacked)
+  constexpr bool STACK = lp_stacked<P, SPLIT>();
+  constexpr int KA = lp_ka<P, SPLIT>();
+  constexpr int FT = MT * KA * 64, NF = (SPLIT != LP_SPLIT_NONE && !STACK) ? 2 : 1;
   __shared__ uint4 buf0[BUF];
   __shared__ uint4 buf1[BUF];
   __shared__ uint4 frag[NF * FT];
@@ -434,30 +467,35 @@ __global__ void __launch_bounds__(64 * ((P + 15) / 16), 2) lp_kernel(const LPDev
   if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) scale[0] = (double)m0;
   const double vinv = 1.0 / (double)m0;
   // Each element once per workgroup: v[(p P + q) DP + s] is element e of lane (q % 16) + 16 g of fragment
-  // (m = s NT + q / 16, t = p / 32), where p % 32 = lp_kidx(0, g, e) (hi + lo in the split modes).
+  // (m = s NT + q / 16, t = k / 32), where k % 32 = lp_kidx(0, g, e) for K index k = p (hi + lo in the
+  // split modes; stacked: hi at k = p and lo at k = P + p of the one fragment set).
   T* fh = reinterpret_cast<T*>(frag);
+  auto frag_index = [](int m, int kx, int q) {
+    const int kk = kx & 31, g = (kk & 15) >> 2, e = (kk & 3) + ((kk >> 4) << 2);
+    return (((m * KA + (kx >> 5)) * 64) + (q & 15) + 16 * g) * 8 + e;
+  };
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     const int i = threadIdx.x + NTH * k;
     if (i < NY) {
-      const int sv = i % DP, pq = i / DP, q = pq % P, p = pq / P, kk = p & 31;
-      const int m = sv * NT + (q >> 4), g = (kk & 15) >> 2, e = (kk & 3) + ((kk >> 4) << 2);
-      const int idx = (((m * KT + (p >> 5)) * 64) + (q & 15) + 16 * g) * 8 + e;
+      const int sv = i % DP, pq = i / DP, q = pq % P, p = pq / P;
+      const int m = sv * NT + (q >> 4);
       const float x = (float)(xv[k] * vinv);
-      fh[idx] = LT::cvt(x);
-      if (NF == 2) fh[FT * 8 + idx] = LT::cvt(x - LT::round(x));
+      fh[frag_index(m, p, q)] = LT::cvt(x);
+      if (STACK) fh[frag_index(m, P + p, q)] = LT::cvt(x - LT::round(x));
+      else if (NF == 2) fh[FT * 8 + frag_index(m, p, q)] = LT::cvt(x - LT::round(x));
     }
   }
   __syncthreads();
   const V* fv = reinterpret_cast<const V*>(frag);
-  V vt[MT][KT], vl[MT][KT];
+  V vt[MT][KA], vl[MT][KT];
 #pragma unroll
-  for (int m = 0; m < MT; ++m)
+  for (int m = 0; m < MT; ++m) {
 #pragma unroll
-    for (int t = 0; t < KT; ++t) {
-      vt[m][t] = fv[(m * KT + t) * 64 + lane];
-      vl[m][t] = (NF == 2) ? fv[FT + (m * KT + t) * 64 + lane] : V{};
-    }
+    for (int t = 0; t < KA; ++t) vt[m][t] = fv[(m * KA + t) * 64 + lane];
+#pragma unroll
+    for (int t = 0; t < KT; ++t) vl[m][t] = (NF == 2) ? fv[FT + (m * KT + t) * 64 + lane] : V{};
+  }
   f32x4 y[NT][DP];
 #pragma unroll
   for (int j = 0; j < NT; ++j)

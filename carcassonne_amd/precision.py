@@ -152,6 +152,7 @@ class MixedSolver:
         k, rho, prev = 0, None, None
         fp64_passes = 0
         history = []
+        stopped_by = None
         for outer in range(max_outer):
             # FP64 H and N on the newest basis vector, then the projected pencil
             _lib.call("cc_stage3_plan_apply", stream(), self._hplan, ptr(U[k]), ptr(HU[k]))
@@ -173,10 +174,12 @@ class MixedSolver:
             if prev is not None:
                 delta = abs(prev - rho)
                 if delta <= tol_energy * abs(rho):
+                    stopped_by = "energy change"
                     break
                 if len(history) >= 3:
                     q = delta / max(abs(history[-2] - history[-3]), 1e-300)
                     if q < 1.0 and delta * q / (1.0 - q) <= tol_predicted * abs(rho):
+                        stopped_by = "predicted error"
                         break
             prev = rho
             if outer == max_outer - 1:
@@ -196,8 +199,9 @@ class MixedSolver:
                 _lib.call("cc_axpby_f64", stream(), n, -1.0, ptr(tmp), 1.0, ptr(t))
             _lib.call("cc_normalize_f64", stream(), n, ptr(t), ptr(U[k]))
         f.check()
-        converged = len(history) >= 2 and abs(history[-1] - history[-2]) <= max(tol_energy, 1e-9) * abs(rho)
+        converged = stopped_by is not None
         return xr, rho, dict(lowprec_iterations=lp_iters, outer_steps=k, fp64_passes=fp64_passes, converged=converged,
+                             stopped_by=stopped_by or "max_outer",
                              lowprec_ritz=float(res[0]), lowprec_residual=float(res[1]), history=history)
 
     def solve_fixed(self, x0, lp_iters=60, outer=3, inner_iters=20):

@@ -68,6 +68,7 @@ def test_mixed_solve_at_d6_chi64_reaches_oracle_energy(gpu, key, dtype):
     print(key, dtype, "energy", rho, "oracle", g["ground_energy"], "rel err", err, info, flush=True)
     assert solver.Hlp.chunks == (1 if g["model"] == "ising" else 2)
     assert err < 1e-10, (key, dtype, rho, g["ground_energy"], err, info)
+    assert info["converged"] and info["stopped_by"] != "max_outer", info
 
 
 @pytest.mark.parametrize("key", ["D4_chi32_ising_seed0", "D4_chi32_heisenberg_seed0"])
