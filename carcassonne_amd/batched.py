@@ -1,14 +1,20 @@
 """The batched form: many independent systems of uneven (D, chi) and fields solved by one HIP-graph launch.
 
-No reference counterpart (the reference solves one system at a time).  Each system gets its stage-1/2
-operators, its normalization factor and its low-precision plan eagerly (`prepare`); the eigensolves of
-all systems -- fixed-work low-precision Lanczos plus FP64 Rayleigh-Ritz refinement, no host reads
-(MixedSolver.solve_fixed) -- are captured once into one HIP graph, each system on its own stream branch
-forked from the capture stream, so one `replay` launches every solve and the GPU overlaps the small
-systems' kernels.  An autotuner picks, per shape bucket, the low-precision kernel's workgroup count by
-timing candidates on the bucket's first system, and the fixed work (Lanczos steps, refinement passes)
-from one adaptive solve per bucket with a safety margin.  The plain FP64 solve of every system is the
-reference for the batched energies.
+No reference counterpart (the reference solves one system at a time).  `prepare` allocates every
+system's buffers and plans (stage-1/2 operators, normalization factor, FP64 packed slabs, low-precision
+slabs and plan) and runs the setup once eagerly.  `capture` records every system's solve into one HIP
+graph, each system on its own stream branch forked from the capture stream, so one `replay` launches every
+solve and the GPU overlaps the small systems' kernels.  With setup=True (the default) each branch first
+rebuilds its system's setup from the environment tensors into those buffers -- stage 1/2, the
+normalization factor, the FP64 and low-precision slabs -- so a replay is a whole solve of every system,
+and `load_environments` + `replay` solves new environments of the same shapes; with setup=False only the
+eigensolves are recorded.  The eigensolve is the fixed-work mixed solve (low-precision Lanczos plus FP64
+Rayleigh-Ritz refinement, no host reads: MixedSolver.solve_fixed), which also writes a convergence record
+per system; `convergence` reads the records after a replay and `resolve_unconverged` solves the flagged
+systems again with the adaptive solve.  An autotuner picks, per shape bucket, the low-precision kernel's
+workgroup count by timing candidates on the bucket's first system, and the fixed work (Lanczos steps,
+refinement passes) from adaptive solves of every system of the bucket with a safety margin.  The plain
+FP64 solve of every system is the reference for the batched energies.
 """
 import time
 
@@ -120,26 +126,49 @@ class BatchedSolve:
             self.tuning[key]["outer"] = passes + margin_outer
         synchronize()
 
-    def _solve_all(self, streams):
+    def _solve_all(self, streams, setup):
         cur = torch.cuda.current_stream()
-        outs = []
-        for sysm, st in zip(self.systems, streams):
+        outs, keep = [], []
+        for sysm, st, (corners, sides, opc, _) in zip(self.systems, streams, self.inputs):
             st.wait_stream(cur)
             with torch.cuda.stream(st):
+                if setup:
+                    H, N = S2.formExpectationAndNormalizationMultipliers(corners, sides, opc)
+                    sysm["solver"].refresh(H.operator, N.operator)
+                    keep.append((H, N))
                 t = self.tuning[sysm["key"]]
                 outs.append(sysm["solver"].solve_fixed(sysm["x0"], lp_iters=t["lp_iters"], outer=t["outer"]))
         for st in streams:
             cur.wait_stream(st)
+        self._setup_outputs = keep
         return outs
 
-    def capture(self):
-        """Capture every system's eigensolve into one HIP graph (each system on its own stream branch)."""
+    def capture(self, setup=True):
+        """Capture every system's solve into one HIP graph (each system on its own stream branch); with
+        setup, each branch rebuilds its system's setup from the environment tensors first."""
+        self.setup_in_graph = bool(setup)
         self._streams = [torch.cuda.Stream() for _ in self.systems]
-        self._solve_all(self._streams)            # warm-up outside the graph: first-use allocations
+        self._solve_all(self._streams, setup)      # warm-up outside the graph: first-use allocations
         synchronize()
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
-            self.outputs = self._solve_all(self._streams)
+            self.outputs = self._solve_all(self._streams, setup)
+        synchronize()
+
+    def load_environments(self, envs):
+        """Copies new environments -- (corners, sides) lists of tag -> host array, the shapes and tags of
+        the loaded ones, as synthetic.make_environment returns -- into the tensors the graph reads; the
+        next replay solves them (needs a graph captured with setup)."""
+        if not getattr(self, "setup_in_graph", False):
+            raise RuntimeError("load_environments needs a graph captured with setup=True")
+        for (corners, sides, _, _), (c_new, s_new) in zip(self.inputs, envs):
+            for dev, host in zip(list(corners) + list(sides), list(c_new) + list(s_new)):
+                for key, arr in host.items():
+                    t = dev[pipeline.synthetic_tag(key)]
+                    if tuple(t.shape) != tuple(arr.shape):
+                        raise ValueError("load_environments needs the loaded shapes")
+                    t.__dict__.pop("_host", None)
+                    t._t.copy_(torch.from_numpy(np.ascontiguousarray(arr)))
         synchronize()
 
     def replay(self):
@@ -163,6 +192,13 @@ class BatchedSolve:
             out.append((bool(delta <= tol_change and resid <= tol_residual), delta, resid))
         return out
 
+    def _fresh_operators(self, i):
+        """System i's (H, N) built eagerly from its current environment tensors (new stage-2 tensors, so no
+        cached packing or conversion of an earlier environment is reused)."""
+        corners, sides, opc, _ = self.inputs[i]
+        H, N = S2.formExpectationAndNormalizationMultipliers(corners, sides, opc)
+        return H.operator, N.operator
+
     def resolve_unconverged(self, conv=None):
         """The guard's fallback: every system the record flags is solved again eagerly by the adaptive
         mixed solve with the FP16 operator (its energy replaces the graph's).  Returns {index: energy}."""
@@ -171,18 +207,18 @@ class BatchedSolve:
         for i, (ok, _, _) in enumerate(conv):
             if ok:
                 continue
-            sysm = self.systems[i]
-            solver = MixedSolver(sysm["H"].operator, sysm["N"].operator, dtype="f16", factor=sysm["solver"].factor)
-            _, rho, _ = solver.solve(sysm["x0"])
+            H, N = self._fresh_operators(i)
+            _, rho, _ = MixedSolver(H, N, dtype="f16").solve(self.systems[i]["x0"])
             out[i] = rho
         return out
 
     def reference_energies(self):
-        """Plain FP64 solve of every system (the reference energies)."""
+        """Plain FP64 solve of every system's current environment (the reference energies)."""
+        from .solver import GroundStateSolver
         out = []
-        for sysm in self.systems:
-            from .solver import GroundStateSolver
-            gs = GroundStateSolver(sysm["H"].operator, sysm["N"].operator, factor=sysm["solver"].factor)
+        for i, sysm in enumerate(self.systems):
+            H, N = self._fresh_operators(i)
+            gs = GroundStateSolver(H, N)
             x, _, _, _ = gs.solve(sysm["x0"])
             out.append(gs.rayleigh_quotient(x))
         return out

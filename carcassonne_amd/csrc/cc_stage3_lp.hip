@@ -70,7 +70,8 @@ struct LPDevPlan {
   int term_slot[LP_MAX_TERMS];     // LDS slot of each (chunk-ordered) term's B slab
   int term_a[LP_MAX_TERMS];        // the term's A and B channel indices (input order = chunk order)
   int term_b[LP_MAX_TERMS];
-  float coef[LP_MAX_TERMS][LP_MAX_D * LP_MAX_D];
+  float coef[LP_MAX_TERMS][LP_MAX_D * LP_MAX_D];       // scaled by the channel scales (read by the kernel)
+  float coef_raw[LP_MAX_TERMS][LP_MAX_D * LP_MAX_D];   // as given (the source of every rescale)
 };
 
 template <typename T> struct LPType;
@@ -721,6 +722,7 @@ struct LPHandle {
   int64_t P, d;
   int dtype, groups, chunks, split;
   int diagf;           // 1 when every term's coefficient matrix is diagonal or anti-diagonal (d = 2)
+  int n_terms, n_a;    // term count and A-channel count (the scale layout: A channels, then B channels)
   float* work;         // lp_layout(P, d, groups * LP_MAX_CHUNKS); the device plan sits at its start
   LPDevPlan* host;     // the host plan, kept until destroy (source of the stream-ordered upload)
   hipEvent_t uploaded; // recorded after the upload; destroy waits for it before freeing `host`
@@ -770,11 +772,12 @@ int dispatch_lp(hipStream_t st, const LPHandle* h, const double* v, double* y) {
 }
 
 // Host: cut the (A-sorted) term table into chunks whose distinct slabs fit one LDS buffer.
-// coef[t] *= scale[a(t)] * scale[n_a + b(t)] on the device (channel scales computed by the conversion)
+// coef[t] = coef_raw[t] * scale[a(t)] * scale[n_a + b(t)] on the device (channel scales computed by the
+// conversion); idempotent, so slabs converted again into the same buffers only need this kernel again
 __global__ void lp_scale_coef_kernel(LPDevPlan* plan, int n_terms, int dd, int n_a, const double* __restrict__ scale) {
   for (int i = threadIdx.x; i < n_terms * dd; i += blockDim.x) {
     const int t = i / dd, e = i % dd;
-    plan->coef[t][e] = (float)(scale[plan->term_a[t]] * scale[n_a + plan->term_b[t]] * (double)plan->coef[t][e]);
+    plan->coef[t][e] = (float)(scale[plan->term_a[t]] * scale[n_a + plan->term_b[t]] * (double)plan->coef_raw[t][e]);
   }
 }
 
@@ -947,6 +950,7 @@ extern "C" int cc_stage3_lp_plan_create(void** out, int dtype, int n_a, const vo
     cc_set_last_error("lp plan: the term table needs more chunks than the kernel supports");
     return rc;
   }
+  std::memcpy(dp->coef_raw, dp->coef, sizeof(dp->coef));
   // stream-ordered upload (no host wait); the handle owns the host copy until destroy
   hipStream_t st = cc_stream(stream);
   hipError_t e = hipMemcpyAsync(work, dp, sizeof(LPDevPlan), hipMemcpyHostToDevice, st);
@@ -980,10 +984,22 @@ extern "C" int cc_stage3_lp_plan_create(void** out, int dtype, int n_a, const vo
   h->diagf = d == 2;
   for (int t = 0; t < n_terms; ++t)
     if (lp_term_kind(dp->coef[t], (int)d) == LP_FULL) h->diagf = 0;
+  h->n_terms = n_terms;
+  h->n_a = n_a;
   h->work = work;
   h->host = dp;
   h->uploaded = ev;
   *out = h;
+  return CC_OK;
+}
+
+extern "C" int cc_stage3_lp_plan_rescale(void* stream, void* plan, const double* channel_scales) {
+  if (plan == nullptr || channel_scales == nullptr) { cc_set_last_error("lp plan rescale: null argument"); return CC_ERR_ARG; }
+  const LPHandle* h = static_cast<const LPHandle*>(plan);
+  const LPLayout L = lp_layout(h->P, h->d, (int64_t)h->groups * LP_MAX_CHUNKS);
+  lp_scale_coef_kernel<<<1, 256, 0, cc_stream(stream)>>>(reinterpret_cast<LPDevPlan*>(h->work + L.plan), h->n_terms,
+                                                         (int)(h->d * h->d), h->n_a, channel_scales);
+  CC_LAUNCH_CHECK();
   return CC_OK;
 }
 

@@ -16,23 +16,26 @@ from .solver import GroundStateSolver
 from .tensors._2d import sparse as S2
 
 
-def device_environment(corners, sides, op_center):
-    """Upload a synthetic environment (carcassonne_amd.synthetic: plain-tuple channel keys) as tag ->
-    HipData dicts: ("I",) Identity, ("C",) Complete, ("O",) OneSiteOperator, ("T", k) the side's open
-    TwoSiteOperator(k, CENTER) channel, ("TT", k, dir) the center operator's two-site halves."""
+def synthetic_tag(k):
+    """The tag of a synthetic environment's plain-tuple channel key: ("I",) Identity, ("C",) Complete,
+    ("O",) OneSiteOperator, ("T", k) the side's open TwoSiteOperator(k, CENTER) channel, ("TT", k, dir) the
+    center operator's two-site halves."""
     from .sparse import Identity, Complete, OneSiteOperator, TwoSiteOperator, CENTER
+    if k[0] == "I":
+        return Identity()
+    if k[0] == "C":
+        return Complete()
+    if k[0] == "O":
+        return OneSiteOperator(None)
+    if k[0] == "T":
+        return TwoSiteOperator(k[1], CENTER)
+    return TwoSiteOperator(k[1], k[2], 0)
 
-    def tag(k):
-        if k[0] == "I":
-            return Identity()
-        if k[0] == "C":
-            return Complete()
-        if k[0] == "O":
-            return OneSiteOperator(None)
-        if k[0] == "T":
-            return TwoSiteOperator(k[1], CENTER)
-        return TwoSiteOperator(k[1], k[2], 0)
 
+def device_environment(corners, sides, op_center):
+    """Upload a synthetic environment (carcassonne_amd.synthetic: plain-tuple channel keys, see
+    synthetic_tag) as tag -> HipData dicts."""
+    tag = synthetic_tag
     up = HipData.fromArray
     return ([{tag(k): up(v) for k, v in c.items()} for c in corners],
             [{tag(k): up(v) for k, v in s.items()} for s in sides],

@@ -33,6 +33,22 @@ DTYPES = {"f16": _lib.CC_F16, "bf16": _lib.CC_BF16}
 SPLIT_MODES = {True: 1, "both": 1, "vector": 2, False: 0, "none": 0}
 
 
+def _lp_channels(operator):
+    """Distinct A and B channels of a stage-3 operator in first-use order and its terms (A index, term,
+    B index, site operator) sorted by A channel."""
+    a_ids, b_ids, a_list, b_list, rows = {}, {}, [], [], []
+    for k, (A, B, O) in enumerate(zip(operator._A, operator._B, operator._O)):
+        if id(A) not in a_ids:
+            a_ids[id(A)] = len(a_list)
+            a_list.append(A)
+        if id(B) not in b_ids:
+            b_ids[id(B)] = len(b_list)
+            b_list.append(B)
+        rows.append((a_ids[id(A)], k, b_ids[id(B)], O))
+    rows.sort(key=lambda r: (r[0], r[1]))
+    return a_list, b_list, rows
+
+
 class LowPrecisionOperator:
     def __init__(self, operator, dtype="bf16", groups=None, split="vector"):
         if not operator.use_fused:
@@ -40,16 +56,7 @@ class LowPrecisionOperator:
         self.base, self.dtype, self.split = operator, dtype, {1: "both", 2: "vector", 0: "none"}[SPLIT_MODES[split]]
         self.n, self.n_alpha, self.P, self.d = operator.n, operator.n_alpha, operator.P, operator.d
         code = DTYPES[dtype]
-        a_ids, b_ids, a_list, b_list, rows = {}, {}, [], [], []
-        for k, (A, B, O) in enumerate(zip(operator._A, operator._B, operator._O)):
-            if id(A) not in a_ids:
-                a_ids[id(A)] = len(a_list)
-                a_list.append(A)
-            if id(B) not in b_ids:
-                b_ids[id(B)] = len(b_list)
-                b_list.append(B)
-            rows.append((a_ids[id(A)], k, b_ids[id(B)], O))
-        rows.sort(key=lambda r: (r[0], r[1]))
+        a_list, b_list, rows = _lp_channels(operator)
         # each channel is converted once (read in place from its stage-2 tensor) and kept on the channel,
         # so operators built from the same stage-2 tensors share the rounded slabs
         fresh = [ch for ch in a_list + b_list if dtype not in ch.lowprec]
@@ -67,6 +74,7 @@ class LowPrecisionOperator:
             for i, (ch, buf) in enumerate(zip(fresh, bufs)):
                 ch.lowprec[dtype] = (buf, sc[i, 0:1])          # the scale stays on the device
         self._slabs = [ch.lowprec[dtype][0] for ch in a_list + b_list]
+        self._structure = [(r[0], r[2]) for r in rows], [(ch.n_alpha, ch.P) for ch in a_list + b_list]
         # channel scales gathered on the device; the plan applies them to the unscaled coefficients
         self._scales = torch.cat([ch.lowprec[dtype][1] for ch in a_list + b_list])
         coef = []
@@ -95,6 +103,26 @@ class LowPrecisionOperator:
     def apply_flat(self, x, y):
         _lib.call("cc_stage3_lp_plan_apply", stream(), self.plan, ptr(x), ptr(y))
 
+    def refresh(self, operator):
+        """Converts the channels of `operator` -- the same structure as the one this plan was built for,
+        from new stage-2 tensors -- into this plan's slab buffers and rescales the plan's coefficients with
+        the new channel scales (kernels only, no allocation after the first call: capturable in a HIP
+        graph and replayed for new environments)."""
+        a_list, b_list, rows = _lp_channels(operator)
+        structure = [(r[0], r[2]) for r in rows], [(ch.n_alpha, ch.P) for ch in a_list + b_list]
+        if structure != self._structure:
+            raise ValueError("refresh needs an operator of the structure this plan was built for")
+        if getattr(self, "_refresh_sc", None) is None:
+            self._refresh_sc = empty((len(self._slabs), 513))
+        code = DTYPES[self.dtype]
+        for i, (ch, buf) in enumerate(zip(a_list + b_list, self._slabs)):
+            base, (na0, na1, d0, d1), st = ch.source()
+            exact = base.numel() == na0 * na1 * d0 * d1 * d0 * d1
+            _lib.call("cc_stage3_lp_convert_strided", stream(), code, na0, na1, d0, d1, _lib.i64_array(st),
+                      base.numel() if exact else 0, ptr(base), ptr(self._refresh_sc[i]), ptr(buf))
+        self._scales.copy_(self._refresh_sc[:, 0])
+        _lib.call("cc_stage3_lp_plan_rescale", stream(), self.plan, ptr(self._scales))
+
     def __del__(self):
         if getattr(self, "plan", None) is not None:
             try:
@@ -122,6 +150,14 @@ class MixedSolver:
         self._hplan = h_operator.plan_handle()     # N is applied through the formed N_env (factor.apply_n)
         self._lz_work, self._lz_max = None, 0
         self._jd_work = empty((_lib.query("cc_jd_work_size", self.n),))
+
+    def refresh(self, h_operator, n_operator):
+        """The solve's setup for a new environment of the same structure, into this solver's buffers: the
+        normalization factor, the FP64 packed slabs of the H operator this solver was built with, and the
+        low-precision slabs and coefficients (kernels only: capturable in a HIP graph)."""
+        self.factor.refresh(n_operator)
+        self.H.refresh_fused_from(h_operator)
+        self.Hlp.refresh(h_operator)
 
     def solve(self, x0, max_lowprec_iter=300, check_every=4, inner_iters=20, max_outer=8, tol_energy=1e-13,
               tol_predicted=1e-11, lp_tol_residual=0.0, lp_tol_energy=1e-6):

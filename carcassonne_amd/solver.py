@@ -32,27 +32,37 @@ class NormalizationFactor:
     def __init__(self, n_operator):
         if len(n_operator._A) != 1 or n_operator._O[0] is not None:
             raise ValueError("the normalization operator must be a single identity term")
-        A, B = n_operator._A[0], n_operator._B[0]
-        P, Q, na = n_operator.P, n_operator.Q, n_operator.n_alpha
+        P, Q = n_operator.P, n_operator.Q
         self.n_env, self.d = P * Q, n_operator.d
-        G = empty((P * P, Q * Q))
+        self._shape = (P, Q, n_operator.n_alpha, n_operator.d)
+        self._G = empty((P * P, Q * Q))
+        self.N_env = empty((self.n_env, self.n_env))
+        self.L, self.W = empty(self.N_env.shape), empty(self.N_env.shape)
+        self.Lt, self.Wt = empty(self.N_env.shape), empty(self.N_env.shape)
+        self._info = torch.zeros(1, dtype=torch.int32, device=self.L.device)
+        self._lwork = empty((_lib.query("cc_linalg_work_size", self.n_env),))
+        self.refresh(n_operator)
+
+    def refresh(self, n_operator):
+        """Forms N_env, L, W and their transposes from a normalization operator of this factor's shape into
+        the factor's own buffers (kernels only, no allocation or host work, so the sequence can be captured
+        in a HIP graph and replayed for new environments)."""
+        if len(n_operator._A) != 1 or n_operator._O[0] is not None:
+            raise ValueError("the normalization operator must be a single identity term")
+        if (n_operator.P, n_operator.Q, n_operator.n_alpha, n_operator.d) != self._shape:
+            raise ValueError("refresh needs an operator of the factor's shape {}".format(self._shape))
+        A, B = n_operator._A[0], n_operator._B[0]
+        P, Q, na, _ = self._shape
         # G[(p'p)][(q'q)] = sum_alpha A[alpha][(p'p)] B[alpha][(q'q)]
         _lib.call("cc_gemm_f64", stream(), 1, P * P, Q * Q, na, 1.0, ptr(A._t), 0, 1, P * P, ptr(B._t), 0, Q * Q, 1,
-                  0.0, ptr(G), 0, Q * Q, 1)
-        Nenv = empty((self.n_env, self.n_env))
-        _lib.call("cc_permute", stream(), _lib.CC_F64, ptr(G), ptr(Nenv), 4, _lib.i64_array([P, Q, P, Q]),
+                  0.0, ptr(self._G), 0, Q * Q, 1)
+        _lib.call("cc_permute", stream(), _lib.CC_F64, ptr(self._G), ptr(self.N_env), 4, _lib.i64_array([P, Q, P, Q]),
                   _lib.i64_array([P * Q * Q, Q, Q * Q, 1]))
-        self.N_env = Nenv
-        L = Nenv.clone()
-        info = torch.zeros(1, dtype=torch.int32, device=L.device)
-        work = empty((_lib.query("cc_linalg_work_size", self.n_env),))
-        self.W = empty(L.shape)
-        _lib.call("cc_cholesky_inverse_f64", stream(), self.n_env, ptr(L), ptr(self.W), ptr(info), ptr(work))
-        self._info, self._checked = info, False   # read by check() where the solver synchronizes anyway
-        self.L = L
-        self.Lt = empty(L.shape)
-        _lib.call("cc_transpose_f64", stream(), self.n_env, ptr(L), ptr(self.Lt))
-        self.Wt = empty(L.shape)
+        self.L.copy_(self.N_env)
+        _lib.call("cc_cholesky_inverse_f64", stream(), self.n_env, ptr(self.L), ptr(self.W), ptr(self._info),
+                  ptr(self._lwork))
+        self._checked = False   # the pivot flag is read by check() where the solver synchronizes anyway
+        _lib.call("cc_transpose_f64", stream(), self.n_env, ptr(self.L), ptr(self.Lt))
         _lib.call("cc_transpose_f64", stream(), self.n_env, ptr(self.W), ptr(self.Wt))
 
     def check(self):

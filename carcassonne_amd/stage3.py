@@ -145,8 +145,9 @@ class Stage3Operator:
         nops = len({id(o) for o in self._O if o is not None})
         return na <= 16 and nb <= 16 and len(self._A) <= 48 and nops <= 16
 
-    def _build_fused(self):
-        """Pack every distinct slab channel once and build the term table (sorted by A channel)."""
+    def _channels(self):
+        """Distinct A and B channels in first-use order, the distinct site operators (HipData), and the
+        term table (A index, term, B index, operator index) sorted by A channel."""
         a_ids, b_ids, o_ids = {}, {}, {}
         a_list, b_list, ops = [], [], []
         rows = []
@@ -161,10 +162,29 @@ class Stage3Operator:
             if O is not None:
                 if id(O) not in o_ids:
                     o_ids[id(O)] = len(ops)
-                    ops.append(O.toArray().reshape(-1))
+                    ops.append(O)
                 o = o_ids[id(O)]
             rows.append((a_ids[id(A)], k, b_ids[id(B)], o))
         rows.sort()
+        return a_list, b_list, ops, rows
+
+    def refresh_fused_from(self, other):
+        """Packs the channels of `other` -- an operator of the same structure (the same terms, channel
+        sharing and shapes) built from new stage-2 tensors -- into this operator's packed buffers, so that
+        its fused kernel and native plan apply `other` (kernels only; capturable in a HIP graph)."""
+        self._ensure_fused()
+        a_list, b_list, _, rows = self._channels()
+        oa, ob, _, orows = other._channels()
+        if ([(r[0], r[2], r[3]) for r in rows] != [(r[0], r[2], r[3]) for r in orows] or
+                [(c.n_alpha, c.P) for c in a_list + b_list] != [(c.n_alpha, c.P) for c in oa + ob]):
+            raise ValueError("refresh_fused_from needs an operator of the same structure")
+        for ch, buf in zip(oa + ob, self._fused["A"] + self._fused["B"]):
+            base, (na0, na1, d0, d1), st = ch.source()
+            _lib.call("cc_stage3_pack_strided_f64", stream(), na0, na1, d0, d1, _lib.i64_array(st), ptr(base), ptr(buf))
+
+    def _build_fused(self):
+        """Pack every distinct slab channel once and build the term table (sorted by A channel)."""
+        a_list, b_list, ops, rows = self._channels()
         packs = [ch.packed() for ch in a_list + b_list]
         self._fused = dict(
             A=packs[:len(a_list)], B=packs[len(a_list):],
@@ -172,7 +192,7 @@ class Stage3Operator:
             B_ptrs=_lib.ptr_array([t.data_ptr() for t in packs[len(a_list):]]),
             ta=_lib.i32_array([r[0] for r in rows]), tb=_lib.i32_array([r[2] for r in rows]),
             to=_lib.i32_array([r[3] for r in rows]), n_terms=len(rows), n_a=len(a_list), n_b=len(b_list),
-            n_ops=len(ops), ops=_lib.f64_array([x for o in ops for x in o]),
+            n_ops=len(ops), ops=_lib.f64_array([x for o in ops for x in o.toArray().reshape(-1)]),
             work=empty((max(1, _lib.query("cc_stage3_fused_work_size", self.n_alpha, self.P, self.d)),)))
 
     def _join(self, t, which):

@@ -257,6 +257,12 @@ int cc_stage3_lp_plan_chunks(const void* plan, int* out);
 /* out = 1 when every term's coefficient matrix is diagonal or anti-diagonal (d = 2) and the plan runs the
  * kernel's diagonal-form combine, 0 when some term has a full coefficient matrix. */
 int cc_stage3_lp_plan_diagonal_form(const void* plan, int* out);
+/* Recomputes the plan's device coefficients from the unscaled term coefficients given at creation and
+ * new channel scales (same layout as at creation), on `stream` with one kernel and no host work: after
+ * the plan's slab buffers are converted again from a new environment of the same structure (the
+ * reference rebuilds the stage-3 multipliers for every solve, tensors/_2d/sparse.py:59-71), this makes the
+ * plan apply the new operator; a kernel-only sequence, so it can be captured in a HIP graph. */
+int cc_stage3_lp_plan_rescale(void* stream, void* plan, const double* channel_scales);
 int cc_stage3_lp_plan_apply(void* stream, const void* plan, const double* x, double* y);
 int cc_stage3_lp_plan_destroy(void* plan);
 

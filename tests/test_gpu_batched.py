@@ -1,28 +1,64 @@
-"""The batched form: one HIP graph holding the eigensolves of several systems of different shapes.
+"""The batched form: one HIP graph holding the solves of several systems of different shapes, with the
+setup (stage 1/2, normalization factor, FP64 and low-precision slabs) inside the graph or outside it.
 The graph is replayed several times with eager work in between; every replay reproduces the eager
-fixed-work solves exactly and matches the plain FP64 energies."""
+fixed-work solves exactly and matches the plain FP64 energies, and a graph with the setup inside solves
+new environments copied into its input tensors."""
 import pytest
 
 pytestmark = pytest.mark.gpu
 
+SPECS = [(2, 6, 2.5, 1), (3, 5, 3.1, 2), (2, 8, 2.2, 3), (3, 6, 3.7, 4), (4, 4, 2.9, 5)]
 
-def test_graph_batch_matches_plain_fp64(gpu):
+
+@pytest.mark.parametrize("setup", [True, False])
+def test_graph_batch_matches_plain_fp64(gpu, setup):
     from carcassonne_amd.batched import BatchedSolve
     from carcassonne_amd.device import synchronize
-    specs = [(2, 6, 2.5, 1), (3, 5, 3.1, 2), (2, 8, 2.2, 3), (3, 6, 3.7, 4), (4, 4, 2.9, 5)]
-    b = BatchedSolve(specs, dtype="f16")
+    b = BatchedSolve(SPECS, dtype="f16")
     b.prepare()
     b.tune_work()
     ref = b.reference_energies()
     eager = [float(s["solver"].solve_fixed(s["x0"], lp_iters=b.tuning[s["key"]]["lp_iters"],
                                            outer=b.tuning[s["key"]]["outer"])[1].item()) for s in b.systems]
-    b.capture()
+    b.capture(setup=setup)
     for rep in range(3):
         b.replay()
         got = b.energies()
         assert got == eager, (rep, got, eager)
-        assert len(b.convergence()) == len(specs)
+        assert len(b.convergence()) == len(SPECS)
         for g, r in zip(got, ref):
             assert abs(g - r) <= 1e-10 * abs(r), (g, r)
         b.reference_energies()          # eager work between replays
         synchronize()
+
+
+def test_graph_with_setup_solves_new_environments(gpu):
+    """load_environments copies new environments (other seeds, same shapes) into the graph's input
+    tensors; the next replay rebuilds every setup from them: its energies match the plain FP64 solves of
+    the new environments (systems the convergence record flags are solved again by the fallback), differ
+    from the old ones, and a second replay reproduces the first."""
+    from carcassonne_amd import synthetic as syn
+    from carcassonne_amd.batched import BatchedSolve
+    b = BatchedSolve(SPECS, dtype="f16")
+    b.prepare()
+    b.tune_work(margin_iters=16, margin_outer=3)
+    b.capture(setup=True)
+    b.replay()
+    old = b.energies()
+    envs = []
+    for (D, chi, h, seed) in SPECS:
+        c, s, _ = syn.make_environment(D, chi, model="ising", seed=seed + 500)
+        envs.append((c, s))
+    b.load_environments(envs)
+    ref = b.reference_energies()
+    b.replay()
+    got = b.energies()
+    conv = b.convergence()
+    fixed = b.resolve_unconverged(conv)
+    print("old", old, "new", got, "ref", ref, "refixed", fixed, flush=True)
+    for i, (g, r, o) in enumerate(zip(got, ref, old)):
+        e = fixed.get(i, g)
+        assert abs(e - r) <= 1e-10 * abs(r), (i, e, r)
+        assert abs(r - o) > 1e-6 * abs(o), (i, r, o)
+    b.replay()
+    assert b.energies() == got
