@@ -85,3 +85,55 @@ def test_product_compressor_exact_when_rank_suffices():
     full = np.tensordot(Lt, Rt, ((1, 2, 3), (0, 1, 2)))
     comp = np.einsum("lpqw,pa,qb,abwr->lr", Lt, P, P, Rt)
     np.testing.assert_allclose(comp, full, atol=1e-10 * np.abs(full).max())
+
+
+def test_sign_normalize_columns_rule():
+    """Each column is negated exactly when its entry of largest magnitude (the first of equal ones) is
+    negative; applying the rule twice changes nothing."""
+    from oracle.linalg import sign_normalize_columns
+    U = np.array([[0.1, -0.5, 0.3], [-0.9, 0.5, -0.3], [0.2, 0.1, 0.0]])
+    V = sign_normalize_columns(U)
+    np.testing.assert_array_equal(V[:, 0], -U[:, 0])     # -0.9 is the largest magnitude
+    np.testing.assert_array_equal(V[:, 1], -U[:, 1])     # tie -0.5 / 0.5: the first row decides
+    np.testing.assert_array_equal(V[:, 2], U[:, 2])      # tie 0.3 / -0.3: the first row is positive
+    np.testing.assert_array_equal(sign_normalize_columns(V), V)
+
+
+def test_min_norm_solution_is_the_truncated_pseudo_inverse_solution():
+    """min_norm_solution against the SVD pseudo-inverse (a different method): equal to np.linalg.solve for
+    a well-conditioned square system; for A with singular values 1, 1e-3, 1e-7 and a null space the
+    normal-equation cut-off 1e-10 keeps exactly the singular values above 1e-5 of the largest."""
+    from oracle.linalg import min_norm_solution
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((6, 6)) + 6 * np.identity(6)
+    b = rng.standard_normal(6)
+    np.testing.assert_allclose(min_norm_solution(A, b), np.linalg.solve(A, b), rtol=1e-12, atol=1e-12)
+    U, _ = np.linalg.qr(rng.standard_normal((8, 8)))
+    V, _ = np.linalg.qr(rng.standard_normal((5, 5)))
+    s = np.array([1.0, 1e-3, 1e-7, 0.0, 0.0])
+    A = (U[:, :5] * s) @ V.T
+    b = rng.standard_normal(8)
+    want = np.linalg.pinv(A, rcond=1e-5) @ b
+    got = min_norm_solution(A, b)
+    np.testing.assert_allclose(got, want, rtol=1e-8, atol=1e-10 * np.linalg.norm(want))
+    assert np.linalg.norm(V[:, 2:] .T @ got) < 1e-8 * np.linalg.norm(got)   # nothing along the cut directions
+
+
+def test_compressor_start_spans_the_dominant_ket_subspace():
+    """compressor_start(Lt, k): orthonormal columns spanning the k-dimensional dominant subspace of Lt
+    matricized at its inward ket leg (against eigh of the Gram matrix, a different method), the sign rule
+    on every column, and the same start for -Lt (signs are fixed by the rule, not by the input)."""
+    from oracle.linalg import compressor_start
+    rng = np.random.default_rng(5)
+    Lt = rng.standard_normal((3, 6, 6, 2))
+    k = 4
+    U = compressor_start(Lt, k)
+    np.testing.assert_allclose(U.T @ U, np.identity(k), atol=1e-12)
+    m = np.moveaxis(Lt, 1, 0).reshape(6, -1)
+    w, E = np.linalg.eigh(m @ m.T)
+    Pk = E[:, -k:] @ E[:, -k:].T
+    np.testing.assert_allclose(U @ U.T, Pk, atol=1e-10)
+    for j in range(k):
+        i = int(np.argmax(np.abs(U[:, j])))
+        assert U[i, j] > 0
+    np.testing.assert_allclose(compressor_start(-Lt, k), U, atol=1e-12)
