@@ -476,12 +476,13 @@ class System(BaseSystem):
         operator has a fused kernel, else a dense generalized eigensolve of the formed H and N."""
         from ..solver import GroundStateSolver, dense_lowest_eigenpair
         H, N = self.formExpectationAndNormalizationMultipliers()
-        if not H.operator.use_fused:
+        op = getattr(H, "operator", None)     # absent when no stage-3 term survives (H = 0)
+        if op is None or not op.use_fused:
             x, _ = dense_lowest_eigenpair(H.formMatrix(), N.formMatrix())
             self.setStateCenter(HipData(x.reshape(self.state_center_data.shape)))
             return
-        x, theta, resid, iters = GroundStateSolver(H.operator, N.operator).solve(
-            self.state_center_data, tol_residual=1e-12, tol_energy=0.0, max_iter=H.operator.n)
+        x, theta, resid, iters = GroundStateSolver(op, N.operator).solve(
+            self.state_center_data, tol_residual=1e-12, tol_energy=0.0, max_iter=op.n)
         self.setStateCenter(HipData(x.reshape(self.state_center_data.shape)))
 
     # -- gauge moves (_2d.py:503-549) ------------------------------------------------------------
