@@ -129,6 +129,69 @@ def cpu_baseline(gpu_iterations, sample_iterations=6, reps=5):
                                          solves_per_s={k[:-2]: 1.0 / v for k, v in d4.items()})}
 
 
+def _apply_rate(apply, x, reps):
+    import torch
+    from carcassonne_amd.device import synchronize
+    y = torch.empty_like(x)
+    for _ in range(3):
+        apply(x, y)
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        apply(x, y)
+    synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def isolated_matvecs(op, x0, reps=50):
+    """Back-to-back applications of the stage-3 H operator alone (kernel + partial-sum reduction, inputs
+    resident): the FP64 plan, the FP32 plan and the FP16 plan, in matvecs/s."""
+    from carcassonne_amd import _lib, pipeline
+    from carcassonne_amd.device import ptr, stream
+    from carcassonne_amd.precision import LowPrecisionOperator
+    x = (x0._t if hasattr(x0, "_t") else x0).reshape(-1).contiguous()
+    out = {}
+    for name in ("f64", "f32"):
+        h = op.plan_handle(name)
+        out[name] = 1.0 / _apply_rate(lambda a, b: _lib.call("cc_stage3_plan_apply", stream(), h, ptr(a), ptr(b)), x, reps)
+    lp = LowPrecisionOperator(op, "f16")
+    out["f16"] = 1.0 / _apply_rate(lp.apply_flat, x, reps)
+    return {"matvecs_per_s": out, "flops_per_matvec": pipeline.flops_per_matvec(op),
+            "note": "one application = the fused kernel + the fixed-order FP64 sum of its workgroup partials"}
+
+
+def plain_gpu_d4_chi32(seed=0):
+    """The plain GPU workload: TFIM with h drawn from U[2.5, 3.5] (seeded), D=4, chi=32: isolated H matvecs/s
+    (FP64 and FP32 operands) and the whitened Lanczos to a 1e-10 Ritz residual in both, each from the same
+    seeded start (operators and normalization factor built once, outside the timing)."""
+    import numpy as np
+    import torch
+    from carcassonne_amd import pipeline, synthetic as syn
+    from carcassonne_amd.device import synchronize
+    from carcassonne_amd.solver import GroundStateSolver
+    h = float(np.random.default_rng(seed).uniform(2.5, 3.5))
+    c, s_, st = syn.make_environment(4, 32, model="ising", seed=seed)
+    corners, sides, opc = pipeline.device_environment(c, s_, syn.operator_center("ising", h=h))
+    H, N = pipeline.multipliers(corners, sides, opc, graph=False)
+    x0 = torch.from_numpy(st.ravel()).cuda()
+    res = {"h": h, "isolated": isolated_matvecs(H.operator, x0, reps=100)}
+    factor = None
+    for prec in ("f64", "f32"):
+        gs = GroundStateSolver(H.operator, N.operator, factor=factor, precision=prec)
+        factor = gs.factor
+        gs.solve(x0, tol_residual=1e-10)
+        synchronize()
+        t0 = time.perf_counter()
+        x, theta, resid, iters = gs.solve(x0, tol_residual=1e-10)
+        synchronize()
+        res["lanczos_" + prec] = {"ms": (time.perf_counter() - t0) * 1e3, "iterations": iters, "ritz_value": theta,
+                                  "fp64_rayleigh_quotient": gs.rayleigh_quotient(x)}
+    e64 = res["lanczos_f64"]["fp64_rayleigh_quotient"]
+    res["lanczos_f32"]["rel_diff_vs_f64"] = {k: abs(res["lanczos_f32"][k] - e64) / abs(e64)
+                                             for k in ("ritz_value", "fp64_rayleigh_quotient")}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,6 +277,7 @@ def main():
     out = {}
     if args.form in ("both", "fp64"):
         ms, r, kms, nl = run(lambda: pipeline.solve(corners, sides, op_center, x0), args.steps, args.warmup)
+        r_fp64 = r
         op = r["operator"]
         flops = pipeline.flops_per_matvec(op)
         kernel_ms = kms[0] / max(1, nl[0])
@@ -231,6 +295,10 @@ def main():
                             fp64_kernel_ms=kms[0] / max(1, nl[0]), fp64_launches=nl[0], bytes=lp_bytes,
                             gbs=lp_bytes / (kernel_ms * 1e-3) / 1e9, flops=pipeline.flops_per_matvec(r["operator"]),
                             kernel="lp_kernel<36,2,{},{}-split>".format(lp.dtype, lp.split))
+
+    if "fp64" in out:
+        out["isolated"] = isolated_matvecs(r_fp64["operator"], x0)
+        out["plain_gpu_d4"] = plain_gpu_d4_chi32()
 
     head = out.get("mixed") or out["fp64"]
     form = "mixed-f16" if "mixed" in out else "fp64-lanczos"
@@ -274,6 +342,9 @@ def main():
                             "frac": f["tflops"] / PEAK_FP64_TFLOPS, "traffic": None,
                             "kernel": "fused_f64_kernel<36,2> (stage-3 matvec)",
                             "algorithmic_flops_per_launch": f["flops"]}
+    if "isolated" in out:
+        line["isolated_matvecs_d6_chi64"] = out["isolated"]
+        line["plain_gpu_d4_chi32"] = out["plain_gpu_d4"]
     if not args.no_cpu_baseline:
         its = out["fp64"]["iterations"] if "fp64" in out else 80
         line["cpu_baseline"] = cpu_baseline(its)

@@ -215,6 +215,153 @@ __global__ void __launch_bounds__(256) reduce_partials_kernel(int64_t groups, in
   }
 }
 
+// ---- the same matvec with FP32 operands on v_mfma_f32_16x16x4f32 (FP32 accumulation) ----------------------
+// The f32 result layout differs from the f64 one: D[i][j] sits in lane j + 16(i/4), register i%4.  So that the
+// stage-A result is again the stage-B operand without lane exchanges, the stage-A rows interleave the physical
+// index, idx = 2q + s (d = 2): a lane group's four rows of a 16-row tile are q in {8m + 2g, 8m + 2g + 1} times
+// s in {0, 1}.  Stage B's K step u then takes q = 8(u/2) + 2g + u%2 in lane group g (registers 2(u%2) + s of
+// tile u/2), and the B slabs are packed in that K order; the A slabs keep the f64 packing (K = p = 4u + g).
+template <int P>
+struct Shape32 {
+  static constexpr int NT = (P + 15) / 16;      // 16-wide tiles over p' and q'
+  static constexpr int KSA = (P + 3) / 4;       // stage-A K steps over p
+  static constexpr int Q8 = (P + 7) / 8 * 8;    // q padded to whole 16-row tiles of (q, s)
+  static constexpr int MT = Q8 / 8;             // stage-A row tiles
+  static constexpr int KSB = Q8 / 4;            // stage-B K steps over q
+  static constexpr int SLAB_A = KSA * NT * 64;  // packed floats per alpha
+  static constexpr int SLAB_B = KSB * NT * 64;
+};
+__host__ __device__ inline int f32_b_col(int u, int g) { return 8 * (u >> 1) + 2 * g + (u & 1); }
+
+// role 0 (A slabs): packed[alpha][u][n][lane] = S[16n + lane%16][4u + lane/16]; role 1 (B slabs):
+// packed[alpha][u][n][lane] = S[16n + lane%16][f32_b_col(u, lane/16)]; zero outside the slab
+__global__ void __launch_bounds__(256) pack_f32_kernel(int64_t n_alpha, int role, int NT, int KS, CCSlabGather g,
+                                                       const double* __restrict__ src, float* __restrict__ dst) {
+  extern __shared__ double s[];  // P x (P + 1)
+  const int P = g.P, ld = P + 1;
+  const int per = KS * NT * 64;
+  for (int64_t al = blockIdx.x; al < n_alpha; al += gridDim.x) {
+    cc_slab_load(g, src, al, s, ld, 1.0);
+    __syncthreads();
+    float* out = dst + al * per;
+    for (int r = threadIdx.x; r < per; r += blockDim.x) {
+      const int lane = r & 63, t = r >> 6, n = t % NT, u = t / NT;
+      const int row = 16 * n + (lane & 15), col = role == 0 ? 4 * u + (lane >> 4) : f32_b_col(u, lane >> 4);
+      out[r] = (row < P && col < P) ? (float)s[row * ld + col] : 0.0f;
+    }
+    __syncthreads();
+  }
+}
+
+template <int P>
+__global__ void __launch_bounds__(256) fused_f32_kernel(const FusedPlan plan, const double* __restrict__ v,
+                                                        double* __restrict__ partial) {
+  using S = Shape32<P>;
+  constexpr int NT = S::NT, KSA = S::KSA, MT = S::MT, KSB = S::KSB, SLAB_A = S::SLAB_A, SLAB_B = S::SLAB_B;
+  constexpr int DP = 2, NY = P * P * DP, NVT = MT * KSA * 64;
+  constexpr int NLDS = ((NVT + 1) / 2 > 4 * NY) ? (NVT + 1) / 2 : 4 * NY;
+  __shared__ double lds[NLDS];
+  float* ldsv = reinterpret_cast<float*>(lds);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t gw = (int64_t)blockIdx.x * 4 + wave, nw = (int64_t)gridDim.x * 4;
+  const int64_t a0 = plan.n_alpha * gw / nw, a1 = plan.n_alpha * (gw + 1) / nw;
+  // stage-A A operands V^T[idx][p], idx = 2q + s = 16m + lane%16, p = 4u + lane/16, in lane order
+  for (int e = threadIdx.x; e < NVT; e += 256) {
+    const int l = e & 63, mu = e >> 6, u = mu % KSA, m = mu / KSA;
+    const int idx = 16 * m + (l & 15), q = idx >> 1, sv = idx & 1, p = 4 * u + (l >> 4);
+    ldsv[e] = (q < P && p < P) ? (float)v[(p * P + q) * DP + sv] : 0.0f;
+  }
+  __syncthreads();
+
+  f32x4 y[NT][NT][DP];
+#pragma unroll
+  for (int i = 0; i < NT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+#pragma unroll
+      for (int sv = 0; sv < DP; ++sv) y[i][j][sv] = (f32x4){0.f, 0.f, 0.f, 0.f};
+
+  for (int64_t al = a0; al < a1; ++al) {
+    int t = 0;
+    while (t < plan.n_terms) {
+      const int a = plan.ta[t];
+      const float* Ap = reinterpret_cast<const float*>(plan.A[a]) + al * SLAB_A;
+      const int t_begin = t;
+      int t_end = t;
+      while (t_end < plan.n_terms && plan.ta[t_end] == a) ++t_end;
+#pragma unroll
+      for (int np = 0; np < NT; ++np) {
+        f32x4 w[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) w[m] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int u = 0; u < KSA; ++u) {
+          const float bop = Ap[(u * NT + np) * 64 + lane];
+#pragma unroll
+          for (int m = 0; m < MT; ++m)
+            w[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(ldsv[(m * KSA + u) * 64 + lane], bop, w[m], 0, 0, 0);
+        }
+        for (int tt = t_begin; tt < t_end; ++tt) {
+          const float* Bp = reinterpret_cast<const float*>(plan.B[plan.tb[tt]]) + al * SLAB_B;
+          const int o = plan.to[tt];
+          float oc[DP][DP];
+#pragma unroll
+          for (int i = 0; i < DP; ++i)
+#pragma unroll
+            for (int j = 0; j < DP; ++j) oc[i][j] = (o < 0) ? (i == j ? 1.f : 0.f) : (float)plan.ops[o < 0 ? 0 : o][i * DP + j];
+#pragma unroll
+          for (int u = 0; u < KSB; ++u) {
+            float xs[DP], x[DP];
+#pragma unroll
+            for (int sv = 0; sv < DP; ++sv) xs[sv] = w[u >> 1][2 * (u & 1) + sv];
+#pragma unroll
+            for (int sp = 0; sp < DP; ++sp) {
+              float acc = 0.f;
+#pragma unroll
+              for (int sv = 0; sv < DP; ++sv) acc += oc[sp][sv] * xs[sv];
+              x[sp] = acc;
+            }
+#pragma unroll
+            for (int nq = 0; nq < NT; ++nq) {
+              const float bop = Bp[(u * NT + nq) * 64 + lane];
+#pragma unroll
+              for (int sp = 0; sp < DP; ++sp)
+                y[np][nq][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[sp], bop, y[np][nq][sp], 0, 0, 0);
+            }
+          }
+        }
+      }
+      t = t_end;
+    }
+  }
+  // workgroup reduction in FP64 in a fixed order (rows p' = 16 np + 4 (lane/16) + r in the f32 layout)
+  __syncthreads();
+#pragma unroll
+  for (int np = 0; np < NT; ++np)
+#pragma unroll
+    for (int nq = 0; nq < NT; ++nq)
+#pragma unroll
+      for (int sv = 0; sv < DP; ++sv)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int pp = 16 * np + 4 * (lane >> 4) + r, qq = 16 * nq + (lane & 15);
+          if (pp < P && qq < P) lds[wave * NY + (pp * P + qq) * DP + sv] = (double)y[np][nq][sv][r];
+        }
+  __syncthreads();
+  for (int i = threadIdx.x; i < NY; i += 256)
+    partial[(int64_t)blockIdx.x * NY + i] = ((lds[i] + lds[NY + i]) + lds[2 * NY + i]) + lds[3 * NY + i];
+}
+
+template <int P>
+int launch_fused_f32(hipStream_t st, const FusedPlan& plan, const double* v, double* y, double* work, int groups) {
+  fused_f32_kernel<P><<<groups, 256, 0, st>>>(plan, v, work);
+  CC_LAUNCH_CHECK();
+  const int64_t n = (int64_t)P * P * 2;
+  reduce_partials_kernel<<<(unsigned)((n + 15) / 16), 256, 0, st>>>(groups, n, work, y);
+  CC_LAUNCH_CHECK();
+  return CC_OK;
+}
+
 template <int P, int DP>
 int launch_fused(hipStream_t st, const FusedPlan& plan, const double* v, double* y, double* work, int groups,
                  bool profile = true) {
@@ -246,10 +393,22 @@ struct Stage3PlanHandle {
   FusedPlan plan;
   int64_t P, d;
   int groups;
+  int f32;        // 1: the slabs are FP32 packings (cc_stage3_pack_strided_f32) and the FP32 kernel runs
   double* work;
 };
 
 int apply_plan(hipStream_t st, const Stage3PlanHandle* h, const double* v, double* y) {
+  if (h->f32) {
+    switch (h->P) {
+      case 4: return launch_fused_f32<4>(st, h->plan, v, y, h->work, h->groups);
+      case 9: return launch_fused_f32<9>(st, h->plan, v, y, h->work, h->groups);
+      case 16: return launch_fused_f32<16>(st, h->plan, v, y, h->work, h->groups);
+      case 25: return launch_fused_f32<25>(st, h->plan, v, y, h->work, h->groups);
+      case 36: return launch_fused_f32<36>(st, h->plan, v, y, h->work, h->groups);
+    }
+    cc_set_last_error("fused stage-3: shape not instantiated");
+    return CC_ERR_UNSUPPORTED;
+  }
   switch (h->P) {
     case 4: return launch_fused<4, 2>(st, h->plan, v, y, h->work, h->groups);
     case 9: return launch_fused<9, 2>(st, h->plan, v, y, h->work, h->groups);
@@ -441,6 +600,53 @@ extern "C" int cc_stage3_plan_create(void** out, int n_a, const double* const* A
   h->P = P;
   h->d = d;
   h->groups = fused_groups(n_alpha);
+  h->f32 = 0;
+  h->work = work;
+  *out = h;
+  return CC_OK;
+}
+
+extern "C" int cc_stage3_packed_size_f32(int64_t n_alpha, int64_t P, int role, int64_t* out) {
+  if (n_alpha < 0 || P < 1 || (role != 0 && role != 1)) { cc_set_last_error("pack f32: bad arguments"); return CC_ERR_ARG; }
+  const int64_t NT = (P + 15) / 16, KS = role == 0 ? (P + 3) / 4 : (P + 7) / 8 * 2;
+  *out = n_alpha * KS * NT * 64;
+  return CC_OK;
+}
+
+extern "C" int cc_stage3_pack_strided_f32(void* stream, int role, int64_t na0, int64_t na1, int64_t d0, int64_t d1,
+                                          const int64_t* strides, const double* src, float* packed) {
+  if (role != 0 && role != 1) { cc_set_last_error("pack f32: role must be 0 (A) or 1 (B)"); return CC_ERR_ARG; }
+  CCSlabMap map;
+  int rc = cc_slab_map(na0, na1, d0, d1, strides, &map);
+  if (rc) return rc;
+  const int64_t n_alpha = na0 * na1, P = d0 * d1;
+  if (P > PACK_MAX_P) { cc_set_last_error("pack f32: slab extent above 64"); return CC_ERR_UNSUPPORTED; }
+  if (n_alpha == 0) return CC_OK;
+  const int NT = (int)((P + 15) / 16), KS = role == 0 ? (int)((P + 3) / 4) : (int)((P + 7) / 8 * 2);
+  const unsigned grid = (unsigned)(n_alpha < 16384 ? n_alpha : 16384);
+  const size_t lds = (size_t)P * (P + 1) * sizeof(double);
+  pack_f32_kernel<<<grid, 256, lds, cc_stream(stream)>>>(n_alpha, role, NT, KS, cc_slab_gather(map, d0), src, packed);
+  CC_LAUNCH_CHECK();
+  return CC_OK;
+}
+
+extern "C" int cc_stage3_plan_create_f32(void** out, int n_a, const float* const* A_packed, int n_b,
+                                         const float* const* B_packed, int n_terms, const int* term_a, const int* term_b,
+                                         const int* term_o, int n_ops, const double* host_ops, int64_t n_alpha, int64_t P,
+                                         int64_t d, double* work) {
+  *out = nullptr;
+  if (!cc_stage3_fused_supported(P, d)) { cc_set_last_error("fused stage-3: shape not instantiated"); return CC_ERR_UNSUPPORTED; }
+  if (n_a < 1 || n_a > MAX_CH || n_b < 1 || n_b > MAX_CH) { cc_set_last_error("fused stage-3: plan exceeds kernel limits"); return CC_ERR_UNSUPPORTED; }
+  std::vector<const double*> A(n_a), B(n_b);   // the plan stores slab addresses; the FP32 kernel reads them as float
+  for (int i = 0; i < n_a; ++i) A[i] = reinterpret_cast<const double*>(A_packed[i]);
+  for (int i = 0; i < n_b; ++i) B[i] = reinterpret_cast<const double*>(B_packed[i]);
+  Stage3PlanHandle* h = new Stage3PlanHandle();
+  int rc = fill_plan(h->plan, n_a, A.data(), n_b, B.data(), n_terms, term_a, term_b, term_o, n_ops, host_ops, n_alpha, d);
+  if (rc) { delete h; return rc; }
+  h->P = P;
+  h->d = d;
+  h->groups = fused_groups(n_alpha);
+  h->f32 = 1;
   h->work = work;
   *out = h;
   return CC_OK;

@@ -83,15 +83,18 @@ class NormalizationFactor:
 class GroundStateSolver:
     """Lowest eigenpair of (H, N) for one environment, FP64 on the GPU."""
 
-    def __init__(self, h_operator, n_operator, factor=None):
+    def __init__(self, h_operator, n_operator, factor=None, precision="f64"):
+        """precision "f32": the H matvecs run on the FP32-operand kernel (FP32 accumulation; vectors,
+        orthogonalisation, tridiagonal solve and N stay FP64), the FP32 form of the plain method."""
         if not h_operator.use_fused:
             raise NotImplementedError("the device Lanczos needs an operator with a fused kernel "
                                       "(P = Q in {4, 9, 16, 25, 36}, d = 2)")
         self.H, self.N = h_operator, n_operator
         self.factor = factor if factor is not None else NormalizationFactor(n_operator)
         self.n = h_operator.n
+        self.precision = precision
         h_operator._ensure_fused()
-        self._plan = h_operator.plan_handle()
+        self._plan = h_operator.plan_handle(precision)
         self._work = None
         self._max_iter = 0
 

@@ -58,7 +58,7 @@ class _Slab:
         self.n_alpha = self.na0 * self.na1
         self.P = self.shape[2] * self.shape[3]
         self.strided = self.shape[2] == self.shape[4] and self.shape[3] == self.shape[5]
-        self._joined, self._packed, self.lowprec = None, None, {}
+        self._joined, self._packed, self.lowprec, self._packed_f32 = None, None, {}, {}
 
     @staticmethod
     def of(t, which):
@@ -106,6 +106,20 @@ class _Slab:
             _lib.call("cc_stage3_pack_strided_f64", stream(), na0, na1, d0, d1, _lib.i64_array(st), ptr(base), ptr(buf))
             self._packed = buf
         return self._packed
+
+
+    def packed_f32(self, role):
+        """The channel in the FP32 kernel's operand order (cc_stage3_pack_strided_f32): role 0 as an A
+        channel, role 1 as a B channel."""
+        if role not in self._packed_f32:
+            import torch
+            buf = torch.empty(_lib.query("cc_stage3_packed_size_f32", self.n_alpha, self.P, role), dtype=torch.float32,
+                              device=self._base.device)
+            base, (na0, na1, d0, d1), st = self.source()
+            _lib.call("cc_stage3_pack_strided_f32", stream(), role, na0, na1, d0, d1, _lib.i64_array(st), ptr(base),
+                      ptr(buf))
+            self._packed_f32[role] = buf
+        return self._packed_f32[role]
 
 
 class Stage3Operator:
@@ -202,10 +216,25 @@ class Stage3Operator:
         if self._fused is None:
             self._build_fused()
 
-    def plan_handle(self):
-        """A native plan (cc_stage3_plan_create) for solvers that apply this operator many times."""
+    def plan_handle(self, precision="f64"):
+        """A native plan (cc_stage3_plan_create) for solvers that apply this operator many times; precision
+        "f32" gives the FP32-operand plan (cc_stage3_plan_create_f32) of the same operator."""
         self._ensure_fused()
         f = self._fused
+        if precision == "f32":
+            if f.get("plan_f32") is None:
+                a_list, b_list, _, _ = self._channels()
+                pa = [ch.packed_f32(0) for ch in a_list]
+                pb = [ch.packed_f32(1) for ch in b_list]
+                f["f32_slabs"] = pa + pb
+                h = ctypes.c_void_p()
+                _lib.call("cc_stage3_plan_create_f32", ctypes.byref(h), f["n_a"], _lib.ptr_array([t.data_ptr() for t in pa]),
+                          f["n_b"], _lib.ptr_array([t.data_ptr() for t in pb]), f["n_terms"], f["ta"], f["tb"], f["to"],
+                          f["n_ops"], f["ops"], self.n_alpha, self.P, self.d, ptr(f["work"]))
+                f["plan_f32"] = h
+            return f["plan_f32"]
+        if precision != "f64":
+            raise ValueError("precision must be 'f64' or 'f32', not {!r}".format(precision))
         if f.get("plan") is None:
             h = ctypes.c_void_p()
             _lib.call("cc_stage3_plan_create", ctypes.byref(h), f["n_a"], f["A_ptrs"], f["n_b"], f["B_ptrs"],
@@ -216,11 +245,12 @@ class Stage3Operator:
 
     def __del__(self):
         f = getattr(self, "_fused", None)
-        if f and f.get("plan") is not None:
-            try:
-                _lib.load().cc_stage3_plan_destroy(f["plan"])
-            except Exception:
-                pass
+        for key in ("plan", "plan_f32"):
+            if f and f.get(key) is not None:
+                try:
+                    _lib.load().cc_stage3_plan_destroy(f[key])
+                except Exception:
+                    pass
 
     def apply_flat(self, x, y):
         """y = H x for flat device tensors of length n (torch float64)."""

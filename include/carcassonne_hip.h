@@ -171,6 +171,22 @@ int cc_stage3_matvec_fused_f64_noalloc(void* stream, int n_a, const double* cons
                                        const int* term_b, const int* term_o, int n_ops, const double* host_ops,
                                        int64_t n_alpha, int64_t P, int64_t d, const double* v, double* y);
 
+/* The fused matvec with FP32 operands on v_mfma_f32_16x16x4f32 and FP32 accumulation (the workgroup partials
+ * are summed in FP64): the FP32 form of the plain method (no reference counterpart; the arithmetic is that of
+ * tensors/_2d/dense.py:115-173 summed as sparse.py:129-133).  Slabs are packed once per channel from the
+ * stage-2 tensor in place (as cc_stage3_pack_strided_f64) into FP32 operand order, role 0 for A channels and
+ * role 1 for B channels (the B slabs take the K order in which the stage-A result is the stage-B operand);
+ * cc_stage3_packed_size_f32 gives the floats per channel.  cc_stage3_plan_create_f32 makes a plan that
+ * cc_stage3_plan_apply, cc_lanczos_f64 and cc_stage3_plan_destroy take like an FP64 plan; x and y stay FP64,
+ * `work` holds cc_stage3_fused_work_size doubles. */
+int cc_stage3_packed_size_f32(int64_t n_alpha, int64_t P, int role, int64_t* out);
+int cc_stage3_pack_strided_f32(void* stream, int role, int64_t na0, int64_t na1, int64_t d0, int64_t d1,
+                               const int64_t* strides, const double* src, float* packed);
+int cc_stage3_plan_create_f32(void** out, int n_a, const float* const* A_packed, int n_b,
+                              const float* const* B_packed, int n_terms, const int* term_a, const int* term_b,
+                              const int* term_o, int n_ops, const double* host_ops, int64_t n_alpha, int64_t P,
+                              int64_t d, double* work);
+
 /* Persistent fused-operator plan: validated and copied once, applied many times (the
  * operator of one solve).  `work` (cc_stage3_fused_work_size doubles) stays caller-owned.
  * The plan object holding them is freed by cc_stage3_plan_destroy. */
