@@ -201,8 +201,9 @@ def main():
                     help="absorb: time the fused-bond sweep step (System.contractAndCompressTowards) instead")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="first launch a plain copy kernel (unary_kernel, scale by 1) over 512 MiB three times: "
-                         "its FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic of the solve's kernels")
+                    help="first launch a plain copy kernel (unary_kernel, scale by 1) over 512 MiB three times, then a "
+                         "read of the same 512 MiB through global->LDS loads (cc_calibration_lds_read) three times: "
+                         "their FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic of the solve's kernels")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the solve is a single-GPU kernel path "
@@ -234,8 +235,12 @@ def main():
         for _ in range(3):
             _lib.call("cc_unary_f64", torch.cuda.current_stream().cuda_stream, 3, src.numel(), src.data_ptr(), 1.0,
                       dst.data_ptr())
+        sink = torch.empty(1024, dtype=torch.float32, device="cuda")
+        for _ in range(3):   # the same 512 MiB read through direct global->LDS loads
+            _lib.call("cc_calibration_lds_read", torch.cuda.current_stream().cuda_stream, src.numel() * 8,
+                      src.data_ptr(), sink.data_ptr(), 1024)
         synchronize()
-        del src, dst
+        del src, dst, sink
 
     if args.form == "absorb":
         from carcassonne_amd.system import System

@@ -315,6 +315,40 @@ int cc_prof_stop(hipStream_t st, int kind, hipEvent_t e0) {
   return CC_OK;
 }
 
+// A read of known size through direct global->LDS loads (the path the FP16 stage-3 kernel stages its slabs
+// on): each workgroup takes 1024 consecutive 16-byte pieces per round, each wave four 64-lane instructions
+// into its own LDS rows, waits, and adds one LDS value per thread; one value per workgroup goes to sink so
+// that the loads are observable.  It calibrates the FETCH_SIZE counter for that load path.
+namespace {
+typedef __attribute__((address_space(3))) void cal_lds_void;
+__global__ void __launch_bounds__(256) lds_read_kernel(int64_t n16, const uint4* __restrict__ src, float* __restrict__ sink) {
+  __shared__ uint4 buf[1024];
+  const int wave_base = threadIdx.x & ~63;
+  float acc = 0.0f;
+  for (int64_t base = (int64_t)blockIdx.x * 1024; base < n16; base += (int64_t)gridDim.x * 1024) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      __builtin_amdgcn_global_load_lds(reinterpret_cast<const void*>(src + base + 256 * k + threadIdx.x),
+                                       (cal_lds_void*)(buf + 256 * k + wave_base), 16, 0, 0);
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    __syncthreads();
+    acc += reinterpret_cast<const float*>(buf)[threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sink[blockIdx.x] = acc;
+}
+}  // namespace
+
+extern "C" int cc_calibration_lds_read(void* stream, int64_t n_bytes, const void* src, float* sink, int blocks) {
+  if (n_bytes <= 0 || n_bytes % 16384 != 0 || blocks < 1 || blocks > 65535) {
+    cc_set_last_error("cc_calibration_lds_read: n_bytes must be a positive multiple of 16384 and blocks in [1, 65535]");
+    return CC_ERR_ARG;
+  }
+  lds_read_kernel<<<blocks, 256, 0, cc_stream(stream)>>>(n_bytes / 16, static_cast<const uint4*>(src), sink);
+  CC_LAUNCH_CHECK();
+  return CC_OK;
+}
+
 extern "C" int cc_profile_begin_sampled(int every) {
   if (every < 1) {
     cc_set_last_error("cc_profile_begin_sampled: every must be at least 1");

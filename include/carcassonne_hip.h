@@ -205,6 +205,11 @@ int cc_stage3_plan_destroy(void* plan);
  * no reference counterpart.) */
 int cc_profile_begin(void);
 int cc_profile_begin_sampled(int every);
+/* Reads n_bytes (a multiple of 16384) of src with direct global->LDS loads, the load path of the FP16
+ * stage-3 kernel, over `blocks` workgroups, writing one float per workgroup to sink (blocks floats): a kernel
+ * of known fetched bytes that calibrates the FETCH_SIZE hardware counter for that path.  (Measurement
+ * support; no reference counterpart.) */
+int cc_calibration_lds_read(void* stream, int64_t n_bytes, const void* src, float* sink, int blocks);
 int cc_profile_end(int n_kinds, double* total_ms, int64_t* launches);
 
 /* ---- dense linear algebra of the solve --------------------------------------------
