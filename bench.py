@@ -100,9 +100,9 @@ def cpu_baseline(gpu_iterations, sample_iterations=6, reps=5):
             w -= Qm.T @ (Qm @ w)
             Q.append(w / np.linalg.norm(w))
         t3 = time.perf_counter()
-        return (t1 - t0) + gpu_iterations * (t3 - t2) / sample_iterations, t3 - t0
+        return (t1 - t0) + gpu_iterations * (t3 - t2) / sample_iterations, t3 - t0, t3 - t2
 
-    t_solve, t_sample = headline_sample()
+    t_solve, t_sample, t_steps = headline_sample()
     op4, st4 = _oracle_operator(4, 32, 0)
     H4, N4 = op4.matrix_H(), op4.matrix_N()
     use_dense = OS.is_cheaper_to_form_matrix(6, op4.cost_H, op4.cost_H_form, op4.n)
@@ -120,13 +120,70 @@ def cpu_baseline(gpu_iterations, sample_iterations=6, reps=5):
         with threadpool_limits(1):
             d4[name + "_1_thread_s"] = _median_time(fn, reps)
         d4[name + "_all_threads_s"] = _median_time(fn, reps)
+    d4_mv = {}
+    x4 = st4.ravel()
+    for label, limit in (("1_thread", 1), ("all_threads", None)):
+        if limit is None:
+            d4_mv[label] = 1.0 / _median_time(lambda: op4.apply_H(x4), reps)
+        else:
+            with threadpool_limits(limit):
+                d4_mv[label] = 1.0 / _median_time(lambda: op4.apply_H(x4), reps)
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu_model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": int(threads), "kind": "port",
+            "cpu_model": cpu_model, "env_threads": {k: os.environ.get(k) for k in ("OPENBLAS_NUM_THREADS", "OMP_NUM_THREADS")},
+            "d4_chi32_matvecs_per_s": dict(d4_mv, statistic="median of {} after one warm-up".format(reps)),
+            "d6_chi64_lanczos_steps_per_s_all_threads": sample_iterations / t_steps,
             "sample": "oracle stage 1/2 + Cholesky of N_env + {} Lanczos steps at D=6, chi=64 ({:.1f} s); solve time "
                       "extrapolated to the plain FP64 form's {} iterations".format(sample_iterations, t_sample,
                                                                                  gpu_iterations),
             "host_cpus": len(os.sched_getaffinity(0)),
             "d4_chi32_full_solves": dict(d4, reps=reps, statistic="median after one warm-up",
                                          solves_per_s={k[:-2]: 1.0 / v for k, v in d4.items()})}
+
+
+def measured_peaks():
+    """The machine's own ceilings, measured here beside the published figures the roofline uses: a 512 MiB
+    copy (unary_kernel, read + write) and a 512 MiB read through global->LDS loads (HBM), the same read of an
+    85 MB buffer repeated (resident in the 256 MB Infinity Cache, the size of the FP16 kernel's slabs), and
+    the dense MFMA rates (f16 16x16x32 and f64 16x16x4, eight independent accumulators per wave)."""
+    import torch
+    from carcassonne_amd import _lib
+    st = torch.cuda.current_stream()
+
+    def seconds(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            fn()
+        e1.record(st)
+        e1.synchronize()
+        return e0.elapsed_time(e1) * 1e-3 / reps
+
+    n = COPY_CALIBRATION_DOUBLES
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    sink = torch.empty(1024 * 256, dtype=torch.float32, device="cuda")
+    out = {}
+    t = seconds(lambda: _lib.call("cc_unary_f64", st.cuda_stream, 3, n, src.data_ptr(), 1.0, dst.data_ptr()), 5)
+    out["hbm_copy_gbs"] = 2 * n * 8 / t / 1e9
+    t = seconds(lambda: _lib.call("cc_calibration_lds_read", st.cuda_stream, n * 8, src.data_ptr(), sink.data_ptr(), 1024), 5)
+    out["hbm_read_global_to_lds_gbs"] = n * 8 / t / 1e9
+    small = (85 << 20) // 16384 * 16384
+    t = seconds(lambda: _lib.call("cc_calibration_lds_read", st.cuda_stream, small, src.data_ptr(), sink.data_ptr(), 1024), 20)
+    out["cache_resident_read_85MB_gbs"] = small / t / 1e9
+    for kind, name, iters, flop in ((0, "mfma_f16_tflops", 20000, 16384), (1, "mfma_f64_tflops", 5000, 2048)):
+        t = seconds(lambda: _lib.call("cc_calibration_mfma", st.cuda_stream, kind, iters, 1024, sink.data_ptr()), 3)
+        out[name] = 1024 * 4 * iters * 8 * flop / t / 1e12
+    out["published"] = {"hbm_gbs": PEAK_HBM_GBS, "fp16_dense_tflops": 2500.0, "fp64_dense_tflops": PEAK_FP64_TFLOPS}
+    del src, dst, sink
+    return out
 
 
 def _apply_rate(apply, x, reps):
@@ -350,6 +407,7 @@ def main():
     if "isolated" in out:
         line["isolated_matvecs_d6_chi64"] = out["isolated"]
         line["plain_gpu_d4_chi32"] = out["plain_gpu_d4"]
+    line["measured_peaks"] = measured_peaks()
     if not args.no_cpu_baseline:
         its = out["fp64"]["iterations"] if "fp64" in out else 80
         line["cpu_baseline"] = cpu_baseline(its)

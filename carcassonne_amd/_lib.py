@@ -68,6 +68,7 @@ SIGNATURES = {
     "cc_stage3_lp_plan_rescale": [_vp, _vp, _vp],
     "cc_stage3_packed_size_f32": [_i64, _i64, _i32, _pi64],
     "cc_calibration_lds_read": [_vp, _i64, _vp, _vp, _i32],
+    "cc_calibration_mfma": [_vp, _i32, _i32, _i32, _vp],
     "cc_stage3_pack_strided_f32": [_vp, _i32, _i64, _i64, _i64, _i64, _vp, _vp, _vp],
     "cc_stage3_plan_create_f32": [ctypes.POINTER(ctypes.c_void_p), _i32, _pvp, _i32, _pvp, _i32, _pi32, _pi32, _pi32,
                                   _i32, _pf64, _i64, _i64, _i64, _vp],
