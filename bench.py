@@ -178,7 +178,8 @@ def measured_peaks():
     small = (85 << 20) // 16384 * 16384
     t = seconds(lambda: _lib.call("cc_calibration_lds_read", st.cuda_stream, small, src.data_ptr(), sink.data_ptr(), 1024), 20)
     out["cache_resident_read_85MB_gbs"] = small / t / 1e9
-    for kind, name, iters, flop in ((0, "mfma_f16_tflops", 20000, 16384), (1, "mfma_f64_tflops", 5000, 2048)):
+    for kind, name, iters, flop in ((0, "mfma_f16_tflops", 20000, 16384), (1, "mfma_f64_tflops", 5000, 2048),
+                                    (2, "mfma_f16_k16_tflops", 20000, 8192)):
         t = seconds(lambda: _lib.call("cc_calibration_mfma", st.cuda_stream, kind, iters, 1024, sink.data_ptr()), 3)
         out[name] = 1024 * 4 * iters * 8 * flop / t / 1e12
     out["published"] = {"hbm_gbs": PEAK_HBM_GBS, "fp16_dense_tflops": 2500.0, "fp64_dense_tflops": PEAK_FP64_TFLOPS}

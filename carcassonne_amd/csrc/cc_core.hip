@@ -349,6 +349,7 @@ extern "C" int cc_calibration_lds_read(void* stream, int64_t n_bytes, const void
   return CC_OK;
 }
 
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 // Matrix-core throughput: each wave keeps eight independent accumulators and issues `iters` rounds of eight
 // MFMAs (kind 0: v_mfma_f32_16x16x32_f16, 16384 FLOP each; kind 1: v_mfma_f64_16x16x4f64, 2048 FLOP each),
 // then writes one value per thread to sink (blocks x 256 floats) so that the products are observable.
@@ -371,6 +372,24 @@ __global__ void __launch_bounds__(256) mfma_peak_f16_kernel(int iters, float* __
   for (int j = 0; j < 8; ++j) t += acc[j][0] + acc[j][1] + acc[j][2] + acc[j][3];
   sink[(int64_t)blockIdx.x * 256 + threadIdx.x] = t;
 }
+__global__ void __launch_bounds__(256) mfma_peak_f16k16_kernel(int iters, float* __restrict__ sink) {
+  f16x4 a, b;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    a[e] = (_Float16)(1e-3f * (float)((threadIdx.x + e) & 7));
+    b[e] = (_Float16)(1e-3f * (float)((threadIdx.x * 3 + e) & 7));
+  }
+  f32x4 acc[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) acc[j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  for (int i = 0; i < iters; ++i)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, acc[j], 0, 0, 0);
+  float t = 0.f;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) t += acc[j][0] + acc[j][1] + acc[j][2] + acc[j][3];
+  sink[(int64_t)blockIdx.x * 256 + threadIdx.x] = t;
+}
 __global__ void __launch_bounds__(256) mfma_peak_f64_kernel(int iters, float* __restrict__ sink) {
   const double a = 1e-3 * (double)(threadIdx.x & 7), b = 1e-3 * (double)((threadIdx.x * 3) & 7);
   f64x4 acc[8];
@@ -387,12 +406,14 @@ __global__ void __launch_bounds__(256) mfma_peak_f64_kernel(int iters, float* __
 }  // namespace
 
 extern "C" int cc_calibration_mfma(void* stream, int kind, int iters, int blocks, float* sink) {
-  if ((kind != 0 && kind != 1) || iters < 1 || blocks < 1 || blocks > 65535) {
-    cc_set_last_error("cc_calibration_mfma: kind must be 0 (f16) or 1 (f64), iters >= 1, blocks in [1, 65535]");
+  if (kind < 0 || kind > 2 || iters < 1 || blocks < 1 || blocks > 65535) {
+    cc_set_last_error("cc_calibration_mfma: kind must be 0 (f16 16x16x32), 1 (f64) or 2 (f16 16x16x16), iters >= 1, "
+                      "blocks in [1, 65535]");
     return CC_ERR_ARG;
   }
   if (kind == 0) mfma_peak_f16_kernel<<<blocks, 256, 0, cc_stream(stream)>>>(iters, sink);
-  else mfma_peak_f64_kernel<<<blocks, 256, 0, cc_stream(stream)>>>(iters, sink);
+  else if (kind == 1) mfma_peak_f64_kernel<<<blocks, 256, 0, cc_stream(stream)>>>(iters, sink);
+  else mfma_peak_f16k16_kernel<<<blocks, 256, 0, cc_stream(stream)>>>(iters, sink);
   CC_LAUNCH_CHECK();
   return CC_OK;
 }

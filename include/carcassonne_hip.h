@@ -211,7 +211,8 @@ int cc_profile_begin_sampled(int every);
  * support; no reference counterpart.) */
 int cc_calibration_lds_read(void* stream, int64_t n_bytes, const void* src, float* sink, int blocks);
 /* Matrix-core throughput check: `blocks` workgroups of four waves, each wave issuing `iters` rounds of eight
- * independent MFMAs (kind 0: v_mfma_f32_16x16x32_f16, 16384 FLOP; kind 1: v_mfma_f64_16x16x4f64, 2048 FLOP),
+ * independent MFMAs (kind 0: v_mfma_f32_16x16x32_f16, 16384 FLOP; kind 1: v_mfma_f64_16x16x4f64, 2048 FLOP;
+ * kind 2: v_mfma_f32_16x16x16_f16, 8192 FLOP),
  * writing blocks x 256 floats to sink: confirms the published dense peaks on the machine the figures are
  * priced against.  (Measurement support; no reference counterpart.) */
 int cc_calibration_mfma(void* stream, int kind, int iters, int blocks, float* sink);
