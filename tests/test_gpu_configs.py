@@ -153,3 +153,44 @@ def test_batched_64_systems_match_oracle(gpu, dtype):
         if dtype == "f16":
             assert not flagged
         synchronize()
+
+
+@pytest.mark.parametrize("key", ["D6_chi64_ising_seed0", "D6_chi64_heisenberg_seed0"])
+def test_operator_properties_at_d6_chi64(gpu, key):
+    """Size-independent properties of the stage-3 operators at the headline size, for every kernel: H and
+    N are linear (H(ax + by) = a Hx + b Hy) and symmetric (y.Hx = x.Hy; the environment is ket/bra
+    symmetric), N is positive on random vectors, and the FP32 and FP16 kernels agree with the FP64 one to
+    their rounding (FP32 operands 2e-6, FP16 slabs 3e-3 relative)."""
+    from carcassonne_amd import _lib
+    from carcassonne_amd.device import ptr, stream
+    from carcassonne_amd.precision import LowPrecisionOperator
+    g, H, N, st = _operators(key)
+    rng = np.random.default_rng(7)
+    x = torch.from_numpy(rng.standard_normal(H.n)).cuda()
+    y = torch.from_numpy(rng.standard_normal(H.n)).cuda()
+    a, b = 0.37, -1.9
+    lp = LowPrecisionOperator(H, "f16")
+
+    def f64(op):
+        return lambda v: (lambda out: (op.apply_flat(v, out), out)[1])(torch.empty_like(v))
+
+    def f32(op):
+        h = op.plan_handle("f32")
+        return lambda v: (lambda out: (_lib.call("cc_stage3_plan_apply", stream(), h, ptr(v), ptr(out)), out)[1])(torch.empty_like(v))
+
+    def f16(v):
+        out = torch.empty_like(v)
+        lp.apply_flat(v, out)
+        return out
+
+    for name, apply, tol_lin in (("H f64", f64(H), 1e-13), ("N f64", f64(N), 1e-13), ("H f32", f32(H), 1e-6)):
+        hx, hy, hxy = apply(x), apply(y), apply(a * x + b * y)
+        lin = float((hxy - (a * hx + b * hy)).norm() / hxy.norm())
+        sym = abs(float(y @ hx) - float(x @ hy)) / (float(hx.norm()) * float(y.norm()))
+        print(key, name, "linearity %.1e symmetry %.1e" % (lin, sym), flush=True)
+        assert lin < tol_lin and sym < 10 * tol_lin, (name, lin, sym)
+    nx = f64(N)(x)
+    assert float(x @ nx) > 0 and float(y @ f64(N)(y)) > 0
+    h64 = f64(H)(x)
+    assert float((f32(H)(x) - h64).norm() / h64.norm()) < 2e-6
+    assert float((f16(x) - h64).norm() / h64.norm()) < 3e-3
