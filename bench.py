@@ -259,9 +259,10 @@ def main():
                     help="absorb: time the fused-bond sweep step (System.contractAndCompressTowards) instead")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="first launch a plain copy kernel (unary_kernel, scale by 1) over 512 MiB three times, then a "
-                         "read of the same 512 MiB through global->LDS loads (cc_calibration_lds_read) three times: "
-                         "their FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic of the solve's kernels")
+                    help="first launch a plain copy kernel (unary_kernel, scale by 1) over 512 MiB three times, a read "
+                         "of the same 512 MiB through global->LDS loads (cc_calibration_lds_read) three times and a "
+                         "matrix-core-bound loop (cc_calibration_mfma) three times: their counters calibrate the HBM "
+                         "traffic and the MFMA busy fraction of the solve's kernels")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the solve is a single-GPU kernel path "
@@ -297,8 +298,11 @@ def main():
         for _ in range(3):   # the same 512 MiB read through direct global->LDS loads
             _lib.call("cc_calibration_lds_read", torch.cuda.current_stream().cuda_stream, src.numel() * 8,
                       src.data_ptr(), sink.data_ptr(), 1024)
+        sink_m = torch.empty(1024 * 256, dtype=torch.float32, device="cuda")
+        for _ in range(3):   # a matrix-core-bound loop: what the MFMA busy counters read at the full rate
+            _lib.call("cc_calibration_mfma", torch.cuda.current_stream().cuda_stream, 0, 2000, 1024, sink_m.data_ptr())
         synchronize()
-        del src, dst, sink
+        del src, dst, sink, sink_m
 
     if args.form == "absorb":
         from carcassonne_amd.system import System
