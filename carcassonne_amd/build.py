@@ -12,6 +12,13 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-I" + os.path.join(HERE, "..", "include")]
+# -amdgpu-mfma-vgpr-form: MFMA accumulators in ordinary VGPRs.  Without it the register allocator puts
+# them in AGPRs and, in loops that carry several accumulators, copies them between the two register
+# files around the MFMAs (64 copies per 32 MFMAs in the FP64 GEMM's K loop).  Not for the fused stage-3
+# kernels: the FP64 one at P = 36 needs more than the 256 ordinary VGPRs for its nine tiles' accumulators,
+# and the forced form made it 3.3x slower (0.28 -> 0.93 ms per matvec at D = 6, chi = 64).
+VGPR_FORM = ["-mllvm", "-amdgpu-mfma-vgpr-form"]
+VGPR_FORM_EXCLUDED = {"cc_stage3_fused.hip"}
 
 
 def sources():
@@ -20,7 +27,8 @@ def sources():
 
 def _compile(src):
     obj = os.path.join(OUT_DIR, os.path.basename(src)[:-4] + ".o")
-    cmd = [HIPCC] + FLAGS + ["-c", src, "-o", obj]
+    extra = [] if os.path.basename(src) in VGPR_FORM_EXCLUDED else VGPR_FORM
+    cmd = [HIPCC] + FLAGS + extra + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed for {}:\n{}".format(src, r.stderr))

@@ -354,6 +354,12 @@ typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 // MFMAs (kind 0: v_mfma_f32_16x16x32_f16, 16384 FLOP each; kind 1: v_mfma_f64_16x16x4f64, 2048 FLOP each),
 // then writes one value per thread to sink (blocks x 256 floats) so that the products are observable.
 namespace {
+// Distinct starting accumulators: with equal starts and operands the eight chains are one computation,
+// which the compiler merges, and the merged loop copies accumulators around every MFMA.
+__device__ inline f32x4 peak_start_f32(int j) {
+  const float s0 = (float)(j + 1) * (float)threadIdx.x;
+  return (f32x4){s0, s0, s0, s0};
+}
 __global__ void __launch_bounds__(256) mfma_peak_f16_kernel(int iters, float* __restrict__ sink) {
   f16x8 a, b;
 #pragma unroll
@@ -363,7 +369,7 @@ __global__ void __launch_bounds__(256) mfma_peak_f16_kernel(int iters, float* __
   }
   f32x4 acc[8];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) acc[j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  for (int j = 0; j < 8; ++j) acc[j] = peak_start_f32(j);
   for (int i = 0; i < iters; ++i)
 #pragma unroll
     for (int j = 0; j < 8; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, acc[j], 0, 0, 0);
@@ -381,7 +387,7 @@ __global__ void __launch_bounds__(256) mfma_peak_f16k16_kernel(int iters, float*
   }
   f32x4 acc[8];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) acc[j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  for (int j = 0; j < 8; ++j) acc[j] = peak_start_f32(j);
   for (int i = 0; i < iters; ++i)
 #pragma unroll
     for (int j = 0; j < 8; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, acc[j], 0, 0, 0);
@@ -394,7 +400,10 @@ __global__ void __launch_bounds__(256) mfma_peak_f64_kernel(int iters, float* __
   const double a = 1e-3 * (double)(threadIdx.x & 7), b = 1e-3 * (double)((threadIdx.x * 3) & 7);
   f64x4 acc[8];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) acc[j] = (f64x4){0.0, 0.0, 0.0, 0.0};
+  for (int j = 0; j < 8; ++j) {
+    const double s0 = (double)(j + 1) * (double)threadIdx.x;
+    acc[j] = (f64x4){s0, s0, s0, s0};
+  }
   for (int i = 0; i < iters; ++i)
 #pragma unroll
     for (int j = 0; j < 8; ++j) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[j], 0, 0, 0);
