@@ -37,3 +37,27 @@ def ptr(t):
 
 def synchronize():
     torch.cuda.synchronize(device())
+
+
+_SIDE = {}
+
+
+def side_stream():
+    """A second stream (default priority) on the current device, for setup work that runs beside the
+    caller's stream: fork() starts it after everything the caller's stream has queued, join() makes the
+    caller's stream wait for it."""
+    d = device()
+    s = _SIDE.get(d.index)
+    if s is None:
+        s = _SIDE[d.index] = torch.cuda.Stream(d)
+    return s
+
+
+def fork():
+    side = side_stream()
+    side.wait_stream(torch.cuda.current_stream())
+    return side
+
+
+def join(side):
+    torch.cuda.current_stream().wait_stream(side)

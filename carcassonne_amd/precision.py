@@ -24,7 +24,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .device import empty, ptr, stream
+from .device import empty, join, ptr, stream
 from .solver import NormalizationFactor
 
 DTYPES = {"f16": _lib.CC_F16, "bf16": _lib.CC_BF16}
@@ -143,13 +143,18 @@ class MixedSolver:
 
     def __init__(self, h_operator, n_operator, dtype="bf16", factor=None, groups=None, split="vector"):
         self.H, self.N = h_operator, n_operator
+        side = None
+        if factor is None:
+            factor, side = NormalizationFactor.beside(n_operator)
+        self.factor = factor
         self.Hlp = LowPrecisionOperator(h_operator, dtype, groups=groups, split=split)
-        self.factor = factor if factor is not None else NormalizationFactor(n_operator)
         self.n = h_operator.n
         h_operator._ensure_fused()
         self._hplan = h_operator.plan_handle()     # N is applied through the formed N_env (factor.apply_n)
         self._lz_work, self._lz_max = None, 0
         self._jd_work = empty((_lib.query("cc_jd_work_size", self.n),))
+        if side is not None:
+            join(side)
 
     def refresh(self, h_operator, n_operator):
         """The solve's setup for a new environment of the same structure, into this solver's buffers: the

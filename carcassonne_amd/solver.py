@@ -22,14 +22,15 @@ from scipy.linalg import eig
 
 from . import _lib
 from .data import HipData
-from .device import empty, ptr, stream, synchronize
+from .device import empty, fork, join, ptr, stream, synchronize
 from .errors import RelaxFailed
 
 
 class NormalizationFactor:
     """N_env = L L^T and W = L^-1 for a normalization Stage3Operator (one identity term)."""
 
-    def __init__(self, n_operator):
+    def __init__(self, n_operator, defer=False):
+        """defer: allocate only; the caller runs refresh(n_operator) (on a stream of its choice)."""
         if len(n_operator._A) != 1 or n_operator._O[0] is not None:
             raise ValueError("the normalization operator must be a single identity term")
         P, Q = n_operator.P, n_operator.Q
@@ -41,7 +42,20 @@ class NormalizationFactor:
         self.Lt, self.Wt = empty(self.N_env.shape), empty(self.N_env.shape)
         self._info = torch.zeros(1, dtype=torch.int32, device=self.L.device)
         self._lwork = empty((_lib.query("cc_linalg_work_size", self.n_env),))
-        self.refresh(n_operator)
+        if not defer:
+            self.refresh(n_operator)
+
+    @classmethod
+    def beside(cls, n_operator):
+        """The factor with its kernels (the N_env product and the Cholesky recursion, whose leaves occupy
+        one workgroup each) on the side stream, so that the caller's own setup kernels run meanwhile.
+        Returns (factor, side stream); the caller joins the side stream before using the factor."""
+        f = cls(n_operator, defer=True)    # buffers from the caller's stream
+        n_operator._A[0]._t, n_operator._B[0]._t   # the joined slabs too: nothing is allocated on the side stream
+        side = fork()
+        with torch.cuda.stream(side):
+            f.refresh(n_operator)
+        return f, side
 
     def refresh(self, n_operator):
         """Forms N_env, L, W and their transposes from a normalization operator of this factor's shape into
@@ -90,13 +104,18 @@ class GroundStateSolver:
             raise NotImplementedError("the device Lanczos needs an operator with a fused kernel "
                                       "(P = Q in {4, 9, 16, 25, 36}, d = 2)")
         self.H, self.N = h_operator, n_operator
-        self.factor = factor if factor is not None else NormalizationFactor(n_operator)
+        side = None
+        if factor is None:
+            factor, side = NormalizationFactor.beside(n_operator)
+        self.factor = factor
         self.n = h_operator.n
         self.precision = precision
         h_operator._ensure_fused()
         self._plan = h_operator.plan_handle(precision)
         self._work = None
         self._max_iter = 0
+        if side is not None:
+            join(side)
 
     def solve(self, x0, tol_residual=1e-10, tol_energy=1e-13, max_iter=400, check_every=4):
         """x0: flat device tensor (or HipData); returns (x, theta, residual, iterations)."""
