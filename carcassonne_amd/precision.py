@@ -147,14 +147,16 @@ class MixedSolver:
         if factor is None:
             factor, side = NormalizationFactor.beside(n_operator)
         self.factor = factor
-        self.Hlp = LowPrecisionOperator(h_operator, dtype, groups=groups, split=split)
-        self.n = h_operator.n
-        h_operator._ensure_fused()
-        self._hplan = h_operator.plan_handle()     # N is applied through the formed N_env (factor.apply_n)
-        self._lz_work, self._lz_max = None, 0
-        self._jd_work = empty((_lib.query("cc_jd_work_size", self.n),))
-        if side is not None:
-            join(side)
+        try:
+            self.Hlp = LowPrecisionOperator(h_operator, dtype, groups=groups, split=split)
+            self.n = h_operator.n
+            h_operator._ensure_fused()
+            self._hplan = h_operator.plan_handle()     # N is applied through the formed N_env (factor.apply_n)
+            self._lz_work, self._lz_max = None, 0
+            self._jd_work = empty((_lib.query("cc_jd_work_size", self.n),))
+        finally:
+            if side is not None:
+                join(side)
 
     def refresh(self, h_operator, n_operator):
         """The solve's setup for a new environment of the same structure, into this solver's buffers: the

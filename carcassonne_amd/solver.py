@@ -110,12 +110,14 @@ class GroundStateSolver:
         self.factor = factor
         self.n = h_operator.n
         self.precision = precision
-        h_operator._ensure_fused()
-        self._plan = h_operator.plan_handle(precision)
+        try:
+            h_operator._ensure_fused()
+            self._plan = h_operator.plan_handle(precision)
+        finally:
+            if side is not None:
+                join(side)
         self._work = None
         self._max_iter = 0
-        if side is not None:
-            join(side)
 
     def solve(self, x0, tol_residual=1e-10, tol_energy=1e-13, max_iter=400, check_every=4):
         """x0: flat device tensor (or HipData); returns (x, theta, residual, iterations)."""
