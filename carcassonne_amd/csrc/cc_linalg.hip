@@ -309,10 +309,22 @@ __global__ void __launch_bounds__(256) skinny_kernel(int64_t n, int tri, const d
   }
 }
 
-__global__ void transpose_kernel(int64_t n, const double* __restrict__ A, double* __restrict__ B) {
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += (int64_t)gridDim.x * blockDim.x) {
-    int64_t i = e / n, j = e % n;
-    B[j * n + i] = A[e];
+// B = A^T for n x n row-major matrices: one 32 x 32 tile per workgroup (32 x 8 threads), read and written
+// along rows through an LDS tile padded to 33 columns.
+__global__ void __launch_bounds__(256) transpose_kernel(int64_t n, const double* __restrict__ A, double* __restrict__ B) {
+  __shared__ double tile[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int64_t r0 = (int64_t)blockIdx.y * 32, c0 = (int64_t)blockIdx.x * 32;
+#pragma unroll
+  for (int k = 0; k < 32; k += 8) {
+    const int64_t r = r0 + ty + k, c = c0 + tx;
+    if (r < n && c < n) tile[ty + k][tx] = A[r * n + c];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 32; k += 8) {
+    const int64_t r = c0 + ty + k, c = r0 + tx;   // B[r][c] = A[c][r]
+    if (r < n && c < n) B[r * n + c] = tile[tx][ty + k];
   }
 }
 
@@ -439,7 +451,9 @@ extern "C" int cc_trtri_lower_f64(void* stream, int64_t n, const double* L, doub
 
 extern "C" int cc_transpose_f64(void* stream, int64_t n, const double* A, double* B) {
   if (n <= 0) return CC_OK;
-  transpose_kernel<<<cc_blocks(n * n, 256), 256, 0, cc_stream(stream)>>>(n, A, B);
+  const int64_t tiles = (n + 31) / 32;
+  if (tiles > 65535) { cc_set_last_error("transpose: n too large for one launch"); return CC_ERR_UNSUPPORTED; }
+  transpose_kernel<<<dim3((unsigned)tiles, (unsigned)tiles), 256, 0, cc_stream(stream)>>>(n, A, B);
   CC_LAUNCH_CHECK();
   return CC_OK;
 }
