@@ -45,6 +45,29 @@ def test_gemm_tile_shapes_match_numpy(gpu):
         np.testing.assert_allclose(C.cpu().numpy(), want, rtol=1e-12, atol=1e-11 * np.sqrt(K))
 
 
+def test_gemm_batched_match_numpy(gpu):
+    """cc_gemm_f64 batched with strides between the batch entries (the normalization environment's
+    product split over four alpha ranges is 4 x 1296 x 1296 x 1024), K-contiguous and row-contiguous
+    operands, ragged edges, alpha and beta."""
+    import torch
+    from carcassonne_amd import _lib
+    from carcassonne_amd.device import ptr, stream
+    rng = _rng(5)
+    for batch, M, N, K, a_t, b_t in ((4, 1296, 1296, 1024, True, False), (3, 1300, 1290, 600, False, True)):
+        a = rng.standard_normal((batch, K, M) if a_t else (batch, M, K))
+        b = rng.standard_normal((batch, N, K) if b_t else (batch, K, N))
+        c0 = rng.standard_normal((batch, M, N))
+        A, B, C = (torch.from_numpy(x.copy()).cuda() for x in (a, b, c0))
+        sAm, sAk = (1, M) if a_t else (K, 1)
+        sBk, sBn = (1, K) if b_t else (N, 1)
+        _lib.call("cc_gemm_f64", stream(), batch, M, N, K, -0.75, ptr(A), M * K, sAm, sAk, ptr(B), K * N, sBk, sBn,
+                  0.5, ptr(C), M * N, N, 1)
+        at = a.transpose(0, 2, 1) if a_t else a
+        bt = b.transpose(0, 2, 1) if b_t else b
+        want = -0.75 * np.matmul(at, bt) + 0.5 * c0
+        np.testing.assert_allclose(C.cpu().numpy(), want, rtol=1e-12, atol=1e-11 * np.sqrt(K))
+
+
 def test_join_transpose_and_elementwise(gpu):
     from carcassonne_amd.data import HipData
     rng = _rng(2)
