@@ -59,5 +59,13 @@ def fork():
     return side
 
 
-def join(side):
-    torch.cuda.current_stream().wait_stream(side)
+def join(side, produced=()):
+    """The caller's stream waits for `side`; the device tensors of the objects in `produced` (made on
+    `side`, used on the caller's stream from here on) are recorded as in use on the caller's stream, so
+    that the allocator does not hand their memory to new work on `side` before that use is done."""
+    main = torch.cuda.current_stream()
+    main.wait_stream(side)
+    for obj in produced:
+        for v in (vars(obj).values() if hasattr(obj, "__dict__") else ()):
+            if isinstance(v, torch.Tensor) and v.is_cuda:
+                v.record_stream(main)

@@ -402,12 +402,22 @@ class System(BaseSystem):
         if self.just_increased_bandwidth:
             raise InvariantViolatedError("Contracting the current center would blow up the condition number of the "
                                          "normalization matrix;  optimize it or replace it first.")
+        import torch
+        from ..device import fork, join
         i = direction
         iso, _, den = self.state_center_data.normalizeAxis(O(i))
-        corner_l = absorbSparseSideIntoCornerFromLeft(self.corners[i], self.sides[L(i)])
-        corner_r = absorbSparseSideIntoCornerFromRight(self.corners[R(i)], self.sides[R(i)])
-        left = fusedTruncationBasis(corner_l[Identity()], (3, 4, 5), (0, 1, 2), new_dimension)
-        right = fusedTruncationBasis(corner_r[Identity()], (0, 1, 2), (3, 4, 5), new_dimension)
+        # the two corners and their truncation bases are independent: the right one on a side stream
+        # (the bases' 64 x 64 eigen-decompositions run on one workgroup each)
+        side, produced = fork(), []
+        try:
+            with torch.cuda.stream(side):
+                corner_r = absorbSparseSideIntoCornerFromRight(self.corners[R(i)], self.sides[R(i)])
+                right = fusedTruncationBasis(corner_r[Identity()], (0, 1, 2), (3, 4, 5), new_dimension)
+                produced = list(corner_r.values()) + [right]
+            corner_l = absorbSparseSideIntoCornerFromLeft(self.corners[i], self.sides[L(i)])
+            left = fusedTruncationBasis(corner_l[Identity()], (3, 4, 5), (0, 1, 2), new_dimension)
+        finally:
+            join(side, produced)
         self.sides[i] = absorbSparseCenterSOSIntoSideProjected(i, self.sides[i], iso, self.operator_center_tensor,
                                                                left, right, iso.conj())
         self.corners[i] = mapOverSparseData(lambda t: _project_corner(t, (3, 4, 5), left), corner_l)
