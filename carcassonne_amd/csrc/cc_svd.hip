@@ -207,8 +207,8 @@ __global__ void __launch_bounds__(SVD_THREADS) svd_jacobi_kernel(int64_t m, int 
   }
 }
 // Eigen-decomposition of a symmetric n x n matrix (n <= 64) by the parallel cyclic Jacobi method, matrix and
-// eigenvectors in LDS: each round of the round-robin ordering rotates n/2 disjoint (p, q) pairs, first the
-// columns of G and V (G J, V J) and then the rows of G (J^T G J).  A pair is rotated while
+// eigenvectors in LDS: each round of the round-robin ordering rotates n/2 disjoint (p, q) pairs, G -> J^T G J
+// and V -> V J, in one phase over the 2 x 2 blocks that pairs of pairs cut out of G.  A pair is rotated while
 // |g_pq| > 1e-15 sqrt(|g_pp g_qq|) and |g_pq| > 1e-15 tr|G| (off-diagonal entries below the second bound move
 // the eigenvalues by less than the rounding of G itself, and rotating them would not terminate for
 // rank-deficient G).  Outputs: lam descending (stable by index) and Vt[r][i] = component i of eigenvector r.
@@ -267,59 +267,40 @@ __global__ void __launch_bounds__(SYM_THREADS) sym_jacobi_kernel(int n_rt, const
         rot_on[tid] = on;
       }
       __syncthreads();
-      // The tasks of a thread touch disjoint entries, so all of its loads are issued before its stores.
-      {  // columns p, q of G and V
-        int ip[SYM_TASKS], iq[SYM_TASKS];
-        double c[SYM_TASKS], sn[SYM_TASKS], gx[SYM_TASKS], gy[SYM_TASKS], vx[SYM_TASKS], vy[SYM_TASKS];
-        bool act[SYM_TASKS];
+      // One phase: thread t < half^2 takes the 2 x 2 block of G on the rows of pair t / half and the columns
+      // of pair t % half and forms J1^T (B J2) in that order (the column rotation, then the row rotation,
+      // each entry through the same operations as a columns-then-rows sweep of the whole matrix); the V
+      // columns of every pair (V J) are spread over all threads.  Blocks and V entries are disjoint, so
+      // one barrier ends the round.  A pair that does not rotate acts as the identity.
+      if (tid < half * half) {
+        const int k1 = tid / half, k2 = tid - (tid / half) * half;
+        const int a0 = rot_p[k1], a1 = rot_q[k1], b0 = rot_p[k2], b1 = rot_q[k2];
+        const bool on1 = rot_on[k1], on2 = rot_on[k2];
+        if (on1 || on2) {
+          const double c1 = on1 ? rot_c[k1] : 1.0, s1 = on1 ? rot_s[k1] : 0.0;
+          const double c2 = on2 ? rot_c[k2] : 1.0, s2 = on2 ? rot_s[k2] : 0.0;
+          const bool r0 = a0 < n, r1 = a1 < n, q0 = b0 < n, q1 = b1 < n;   // indices past n are padding
+          const double x00 = (r0 && q0) ? G[a0 * n + b0] : 0.0, x01 = (r0 && q1) ? G[a0 * n + b1] : 0.0;
+          const double x10 = (r1 && q0) ? G[a1 * n + b0] : 0.0, x11 = (r1 && q1) ? G[a1 * n + b1] : 0.0;
+          // columns (B J2)
+          const double y00 = c2 * x00 - s2 * x01, y01 = s2 * x00 + c2 * x01;
+          const double y10 = c2 * x10 - s2 * x11, y11 = s2 * x10 + c2 * x11;
+          // rows (J1^T (B J2))
+          if (r0 && q0) G[a0 * n + b0] = c1 * y00 - s1 * y10;
+          if (r0 && q1) G[a0 * n + b1] = c1 * y01 - s1 * y11;
+          if (r1 && q0) G[a1 * n + b0] = s1 * y00 + c1 * y10;
+          if (r1 && q1) G[a1 * n + b1] = s1 * y01 + c1 * y11;
+        }
+      }
+      {  // columns p, q of V
 #pragma unroll
         for (int it = 0; it < SYM_TASKS; ++it) {
           const int e = tid + it * SYM_THREADS, k = e % half, i = e / half;
-          act[it] = e < n * half && rot_on[k];
-          if (act[it]) {
-            ip[it] = i * n + rot_p[k];
-            iq[it] = i * n + rot_q[k];
-            c[it] = rot_c[k];
-            sn[it] = rot_s[k];
-            gx[it] = G[ip[it]];
-            gy[it] = G[iq[it]];
-            vx[it] = V[ip[it]];
-            vy[it] = V[iq[it]];
-          }
-        }
-#pragma unroll
-        for (int it = 0; it < SYM_TASKS; ++it) {
-          if (act[it]) {
-            G[ip[it]] = c[it] * gx[it] - sn[it] * gy[it];
-            G[iq[it]] = sn[it] * gx[it] + c[it] * gy[it];
-            V[ip[it]] = c[it] * vx[it] - sn[it] * vy[it];
-            V[iq[it]] = sn[it] * vx[it] + c[it] * vy[it];
-          }
-        }
-      }
-      __syncthreads();
-      {  // rows p, q of G
-        int ip[SYM_TASKS], iq[SYM_TASKS];
-        double c[SYM_TASKS], sn[SYM_TASKS], gx[SYM_TASKS], gy[SYM_TASKS];
-        bool act[SYM_TASKS];
-#pragma unroll
-        for (int it = 0; it < SYM_TASKS; ++it) {
-          const int e = tid + it * SYM_THREADS, j = e % n, k = e / n;
-          act[it] = e < n * half && rot_on[k];
-          if (act[it]) {
-            ip[it] = rot_p[k] * n + j;
-            iq[it] = rot_q[k] * n + j;
-            c[it] = rot_c[k];
-            sn[it] = rot_s[k];
-            gx[it] = G[ip[it]];
-            gy[it] = G[iq[it]];
-          }
-        }
-#pragma unroll
-        for (int it = 0; it < SYM_TASKS; ++it) {
-          if (act[it]) {
-            G[ip[it]] = c[it] * gx[it] - sn[it] * gy[it];
-            G[iq[it]] = sn[it] * gx[it] + c[it] * gy[it];
+          if (e < n * half && rot_on[k]) {
+            const int ip = i * n + rot_p[k], iq = i * n + rot_q[k];
+            const double c = rot_c[k], sn = rot_s[k], vx = V[ip], vy = V[iq];
+            V[ip] = c * vx - sn * vy;
+            V[iq] = sn * vx + c * vy;
           }
         }
       }
