@@ -48,6 +48,34 @@ __global__ void permute_kernel(const T* __restrict__ src, T* __restrict__ dst, i
   }
 }
 
+// The same for fewer than 2^32 output elements, the output index split into coordinates with 32-bit
+// divisions by the invariant extents done as a multiply-high and shifts (Granlund and Montgomery,
+// "Division by invariant integers using multiplication", 1994: for 1 < d < 2^32, l = ceil(log2 d),
+// m = floor(2^32 (2^l - d) / d) + 1, x / d = (t + ((x - t) >> 1)) >> (l - 1) with t = mulhi(m, x)).
+struct PermDesc32 {
+  int rank;
+  uint32_t shape[CC_MAX_RANK], magic[CC_MAX_RANK], shift[CC_MAX_RANK];
+  int64_t stride[CC_MAX_RANK];
+};
+__device__ __forceinline__ uint32_t perm_div(uint32_t x, uint32_t d, uint32_t m, uint32_t l) {
+  if (d == 1) return x;
+  const uint32_t t = __umulhi(m, x);
+  return (t + ((x - t) >> 1)) >> (l - 1);
+}
+template <typename T>
+__global__ void permute32_kernel(const T* __restrict__ src, T* __restrict__ dst, uint32_t total, PermDesc32 d) {
+  for (uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total; o += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t rem = (uint32_t)o;
+    int64_t off = 0;
+    for (int k = d.rank - 1; k >= 0; --k) {
+      const uint32_t q = perm_div(rem, d.shape[k], d.magic[k], d.shift[k]);
+      off += (int64_t)(rem - q * d.shape[k]) * d.stride[k];
+      rem = q;
+    }
+    dst[o] = src[off];
+  }
+}
+
 static int dtype_size(int dt) {
   switch (dt) {
     case CC_F64: return 8;
@@ -72,6 +100,27 @@ extern "C" int cc_permute(void* stream, int dtype, const void* src, void* dst, i
   }
   if (total == 0) return CC_OK;
   unsigned blocks = cc_blocks(total, 256);
+  if (total < (int64_t)0xFFFFFFFFLL) {
+    PermDesc32 d32;
+    d32.rank = rank;
+    for (int k = 0; k < rank; ++k) {
+      const uint64_t dk = (uint64_t)shape[k];   // >= 1: a zero extent returned above
+      const uint32_t l = dk <= 1 ? 0u : (uint32_t)(64 - __builtin_clzll(dk - 1));
+      d32.shape[k] = (uint32_t)dk;
+      d32.shift[k] = l;
+      d32.magic[k] = dk <= 1 ? 0u : (uint32_t)((((1ull << l) - dk) << 32) / dk + 1);
+      d32.stride[k] = src_strides[k];
+    }
+    const uint32_t t32 = (uint32_t)total;
+    switch (dtype_size(dtype)) {
+      case 8: permute32_kernel<uint64_t><<<blocks, 256, 0, cc_stream(stream)>>>((const uint64_t*)src, (uint64_t*)dst, t32, d32); break;
+      case 4: permute32_kernel<uint32_t><<<blocks, 256, 0, cc_stream(stream)>>>((const uint32_t*)src, (uint32_t*)dst, t32, d32); break;
+      case 2: permute32_kernel<uint16_t><<<blocks, 256, 0, cc_stream(stream)>>>((const uint16_t*)src, (uint16_t*)dst, t32, d32); break;
+      default: cc_set_last_error("cc_permute: bad dtype"); return CC_ERR_ARG;
+    }
+    CC_LAUNCH_CHECK();
+    return CC_OK;
+  }
   switch (dtype_size(dtype)) {
     case 8: permute_kernel<uint64_t><<<blocks, 256, 0, cc_stream(stream)>>>((const uint64_t*)src, (uint64_t*)dst, total, d); break;
     case 4: permute_kernel<uint32_t><<<blocks, 256, 0, cc_stream(stream)>>>((const uint32_t*)src, (uint32_t*)dst, total, d); break;
