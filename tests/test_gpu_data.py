@@ -68,6 +68,31 @@ def test_gemm_batched_match_numpy(gpu):
         np.testing.assert_allclose(C.cpu().numpy(), want, rtol=1e-12, atol=1e-11 * np.sqrt(K))
 
 
+def test_permute_awkward_extents_match_numpy(gpu):
+    """cc_permute (the 32-bit coordinate path, extents divided by multiply-high) on rank-6 shapes with
+    extents of 1, primes and large odd values, and on a strided source (every other element of a base)."""
+    import torch
+    from carcassonne_amd import _lib
+    from carcassonne_amd.device import ptr, stream
+    rng = _rng(6)
+    for shape, perm in (((1, 7, 13, 3, 1, 5), (5, 2, 0, 4, 1, 3)), ((97, 1, 3, 11, 2, 1), (3, 0, 5, 2, 4, 1)),
+                        ((4099, 3, 5), (2, 0, 1)), ((2, 3, 4, 5, 6, 7), (5, 4, 3, 2, 1, 0))):
+        a = rng.standard_normal(shape)
+        st = [int(np.prod(shape[k + 1:])) for k in range(len(shape))]
+        out_shape = [shape[p] for p in perm]
+        src = torch.from_numpy(a.copy()).cuda()
+        dst = torch.empty(int(np.prod(out_shape)), dtype=torch.float64, device="cuda")
+        _lib.call("cc_permute", stream(), _lib.CC_F64, ptr(src), ptr(dst), len(shape), _lib.i64_array(out_shape),
+                  _lib.i64_array([st[p] for p in perm]))
+        np.testing.assert_array_equal(dst.cpu().numpy().reshape(out_shape), a.transpose(perm))
+    # strided source: the even elements of a (6, 10) base read as a (6, 5) matrix, written transposed
+    base = rng.standard_normal((6, 10))
+    src = torch.from_numpy(base.copy()).cuda()
+    dst = torch.empty(30, dtype=torch.float64, device="cuda")
+    _lib.call("cc_permute", stream(), _lib.CC_F64, ptr(src), ptr(dst), 2, _lib.i64_array([5, 6]), _lib.i64_array([2, 10]))
+    np.testing.assert_array_equal(dst.cpu().numpy().reshape(5, 6), base[:, ::2].T)
+
+
 def test_join_transpose_and_elementwise(gpu):
     from carcassonne_amd.data import HipData
     rng = _rng(2)
