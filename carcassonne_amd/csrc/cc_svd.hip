@@ -420,11 +420,18 @@ __global__ void fused_gram_scale_kernel(int64_t far, int64_t k, const double* __
 }
 }  // namespace
 
+// The Gram product G = Cs^T Cs has a far x far output and a sum over m: for m >= 2048 (and far <= 128) it
+// is cut into fused_gram_slices(m) slices summed by one batched product, and the slices are added in
+// order by a second product with a vector of ones (a single product would occupy a handful of
+// workgroups for the whole sum).
+inline int64_t fused_gram_slices(int64_t m, int64_t far) { return (m >= 2048 && far <= 128) ? (m / 64 < 64 ? m / 64 : 64) : 1; }
+
 extern "C" int cc_fused_truncation_work_size(int64_t nx, int64_t na, int64_t far, int64_t* out) {
   const int64_t m = nx * na * na;
   int64_t svd = 0;
   cc_svd_work_size(1, far, far, &svd);
-  *out = svd + m * far + 4 * far * far + far;
+  const int64_t S = fused_gram_slices(m, far);
+  *out = svd + m * far + 4 * far * far + far + (S > 1 ? (S + 1) * far * far + (S + 1) : 0);
   return CC_OK;
 }
 
@@ -450,7 +457,25 @@ extern "C" int cc_fused_truncation_f64(void* stream, int64_t nx, int64_t na, int
   const unsigned blocks = (unsigned)((m * far + 255) / 256 < 65535 ? (m * far + 255) / 256 : 65535);
   fused_symmetrize_kernel<<<blocks, 256, 0, st>>>(nx, na, far, C, Cs);
   CC_LAUNCH_CHECK();
-  int rc = cc_gemm_f64(stream, 1, far, far, m, 1.0, Cs, 0, 1, far, Cs, 0, far, 1, 0.0, G, 0, far, 1);
+  int rc;
+  const int64_t S = fused_gram_slices(m, far);
+  if (S > 1) {
+    double* parts = lam + far;                       // (S + 1) slices of far x far
+    double* ones = parts + (S + 1) * far * far;      // S + 1 ones
+    const int64_t kc = m / S, rest = m - S * kc, np = S + (rest ? 1 : 0);
+    rc = cc_gemm_f64(stream, S, far, far, kc, 1.0, Cs, kc * far, 1, far, Cs, kc * far, far, 1, 0.0, parts, far * far, far, 1);
+    if (rc) return rc;
+    if (rest) {
+      rc = cc_gemm_f64(stream, 1, far, far, rest, 1.0, Cs + S * kc * far, 0, 1, far, Cs + S * kc * far, 0, far, 1, 0.0,
+                       parts + S * far * far, 0, far, 1);
+      if (rc) return rc;
+    }
+    rc = cc_fill_f64(stream, np, 1.0, ones);
+    if (rc) return rc;
+    rc = cc_gemm_f64(stream, 1, 1, far * far, np, 1.0, ones, 0, np, 1, parts, 0, far * far, 1, 0.0, G, 0, far * far, 1);
+  } else {
+    rc = cc_gemm_f64(stream, 1, far, far, m, 1.0, Cs, 0, 1, far, Cs, 0, far, 1, 0.0, G, 0, far, 1);
+  }
   if (rc) return rc;
   if (far == SYM_MAX_N)
     sym_jacobi_kernel<SYM_MAX_N><<<1, SYM_THREADS, 0, st>>>((int)far, G, lam, Vt);

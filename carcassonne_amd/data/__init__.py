@@ -7,11 +7,13 @@ cc_gemm_f64, cc_svd_f64, elementwise and reduction kernels).  Complex data is
 rejected: the accelerated path is real (Hermitian = symmetric), which covers the
 transverse-field Ising model and the Heisenberg model written with iY.
 """
+import ctypes
+
 import numpy as np
 import torch
 
 from .. import _lib
-from ..device import device, empty, ptr, stream
+from ..device import ONES_LEN, device, empty, ones, ptr, stream
 
 __all__ = ["Data", "HipData", "NDArrayData"]
 
@@ -383,8 +385,10 @@ class HipData(Data):
         return out
 
     # -- contraction (cc_gemm_f64) ----------------------------------------------------
-    def contractWith(self, other, self_axes, other_axes):
-        """tensordot(self, other, (self_axes, other_axes)) on the GPU."""
+    def contractWith(self, other, self_axes, other_axes, split_long_sums=False):
+        """tensordot(self, other, (self_axes, other_axes)) on the GPU.  split_long_sums: a product with a
+        small output and a long sum is summed in slices (_gemm); the sum's rounding then differs from the
+        single product's, so only callers that ask for it get it."""
         ssh, osh = self.shape, other.shape
         self_axes, other_axes = tuple(self_axes), tuple(other_axes)
         key = (ssh, osh, self_axes, other_axes)
@@ -413,8 +417,11 @@ class HipData(Data):
             B, sBk, sBn = other._t, 1, K
         else:
             B, sBk, sBn = other._permuted(b_perm), N, 1
-        _lib.call("cc_gemm_f64", stream(), 1, M, N, K, 1.0, ptr(A), 0, sAm, sAk, ptr(B), 0, sBk, sBn, 0.0,
-                  ptr(out), 0, N, 1)
+        if split_long_sums:
+            _gemm(M, N, K, A, sAm, sAk, B, sBk, sBn, out)
+        else:
+            _lib.call("cc_gemm_f64", stream(), 1, M, N, K, 1.0, ptr(A), 0, sAm, sAk, ptr(B), 0, sBk, sBn, 0.0,
+                      ptr(out), 0, N, 1)
         return HipData._wrap(out)
 
     def contractWithAlongAll(self, other):
@@ -525,6 +532,33 @@ HipData.X = _DeviceConstant([[0, 1], [1, 0]], "X")
 HipData.Z = _DeviceConstant([[1, 0], [0, -1]], "Z")
 HipData.iY = _DeviceConstant([[0, 1], [-1, 0]], "iY")
 HipData.Y = _DeviceConstant(None, "Y")
+
+
+def _gemm(M, N, K, A, sAm, sAk, B, sBk, sBn, out):
+    """out (M x N, row-major) = A (M x K, strides sAm, sAk) B (K x N, strides sBk, sBn).  A product with a
+    small output and a long sum (M and N from 32 to 64, K >= 2048: the corner projections of
+    the sweep step at D=6, chi=64 sum over 2304) would occupy a handful of workgroups for its whole
+    length; its sum is cut into up to 64 slices of at least 32 (one batched launch, plus one for the
+    remainder) and the slices are added in order by a second product with a vector of ones."""
+    if not (32 <= M <= 64 and 32 <= N <= 64 and K >= 2048):
+        _lib.call("cc_gemm_f64", stream(), 1, M, N, K, 1.0, ptr(A), 0, sAm, sAk, ptr(B), 0, sBk, sBn, 0.0,
+                  ptr(out), 0, N, 1)
+        return
+    S = min(64, K // 64)
+    kc = K // S
+    rest = K - S * kc
+    parts = empty((S + (1 if rest else 0), M, N))
+    _lib.call("cc_gemm_f64", stream(), S, M, N, kc, 1.0, ptr(A), kc * sAk, sAm, sAk, ptr(B), kc * sBk, sBk, sBn,
+              0.0, ptr(parts), M * N, N, 1)
+    if rest:
+        a_rest = ptr(A).value + S * kc * sAk * 8
+        b_rest = ptr(B).value + S * kc * sBk * 8
+        _lib.call("cc_gemm_f64", stream(), 1, M, N, rest, 1.0, ctypes.c_void_p(a_rest), 0, sAm, sAk,
+                  ctypes.c_void_p(b_rest), 0, sBk, sBn, 0.0, ptr(parts[S]), 0, N, 1)
+    n_parts = parts.shape[0]
+    assert n_parts <= ONES_LEN
+    _lib.call("cc_gemm_f64", stream(), 1, 1, M * N, n_parts, 1.0, ptr(ones()), 0, n_parts, 1, ptr(parts), 0, M * N, 1,
+              0.0, ptr(out), 0, M * N, 1)
 
 
 class PermutedView(HipData):

@@ -6,13 +6,27 @@ import torch
 _DEVICE = None
 
 
+_ONES = None
+ONES_LEN = 128
+
+
 def device():
-    global _DEVICE
+    global _DEVICE, _ONES
     if _DEVICE is None:
         if not torch.cuda.is_available():
             raise RuntimeError("carcassonne_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
-        _DEVICE = torch.device("cuda", torch.cuda.current_device())
+        d = torch.device("cuda", torch.cuda.current_device())
+        # a constant vector of ones, complete before any stream can read it (products that add slices)
+        _ONES = torch.ones(ONES_LEN, dtype=torch.float64, device=d)
+        torch.cuda.synchronize(d)
+        _DEVICE = d
     return _DEVICE
+
+
+def ones():
+    """ONES_LEN ones on the device (read only)."""
+    device()
+    return _ONES
 
 
 _DEVICE_INDEX = None

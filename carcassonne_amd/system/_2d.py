@@ -55,9 +55,9 @@ def _project_corner(t, bond_legs, basis):
     sh = t.shape
     k = basis.shape[1]
     if tuple(bond_legs) == (3, 4, 5):
-        out = t.join((0, 1, 2), (3, 4, 5)).contractWith(basis, (1,), (0,))
+        out = t.join((0, 1, 2), (3, 4, 5)).contractWith(basis, (1,), (0,), split_long_sums=True)
         return out.split(sh[0], sh[1], sh[2], k, 1, 1)
-    out = basis.contractWith(t.join((0, 1, 2), (3, 4, 5)), (0,), (0,))
+    out = basis.contractWith(t.join((0, 1, 2), (3, 4, 5)), (0,), (0,), split_long_sums=True)
     return out.split(k, 1, 1, sh[3], sh[4], sh[5])
 
 
