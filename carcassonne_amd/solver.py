@@ -28,7 +28,6 @@ from .errors import RelaxFailed
 
 class NormalizationFactor:
     """N_env = L L^T and W = L^-1 for a normalization Stage3Operator (one identity term)."""
-    K_SPLIT = 4
 
     def __init__(self, n_operator, defer=False):
         """defer: allocate only; the caller runs refresh(n_operator) (on a stream of its choice)."""
@@ -37,12 +36,7 @@ class NormalizationFactor:
         P, Q = n_operator.P, n_operator.Q
         self.n_env, self.d = P * Q, n_operator.d
         self._shape = (P, Q, n_operator.n_alpha, n_operator.d)
-        # The N_env product has a small output (P^2 x Q^2) and a long sum (n_alpha): its sum is split in
-        # K_SPLIT slices (one batched launch, four times the workgroups), added into slice 0 afterwards.
-        na = n_operator.n_alpha
-        self._ksplit = self.K_SPLIT if na >= 256 * self.K_SPLIT and na % self.K_SPLIT == 0 else 1
-        self._Gs = empty((self._ksplit, P * P, Q * Q))
-        self._G = self._Gs[0]
+        self._G = empty((P * P, Q * Q))
         self.N_env = empty((self.n_env, self.n_env))
         self.L, self.W = empty(self.N_env.shape), empty(self.N_env.shape)
         self.Lt, self.Wt = empty(self.N_env.shape), empty(self.N_env.shape)
@@ -73,14 +67,9 @@ class NormalizationFactor:
             raise ValueError("refresh needs an operator of the factor's shape {}".format(self._shape))
         A, B = n_operator._A[0], n_operator._B[0]
         P, Q, na, _ = self._shape
-        # G[(p'p)][(q'q)] = sum_alpha A[alpha][(p'p)] B[alpha][(q'q)]: slice k of _Gs sums the k-th of
-        # _ksplit contiguous alpha ranges, then slices 1.. are added into slice 0 (= _G) in order
-        S = self._ksplit
-        kc = na // S
-        _lib.call("cc_gemm_f64", stream(), S, P * P, Q * Q, kc, 1.0, ptr(A._t), kc * P * P, 1, P * P, ptr(B._t),
-                  kc * Q * Q, Q * Q, 1, 0.0, ptr(self._Gs), P * P * Q * Q, Q * Q, 1)
-        for k in range(1, S):
-            _lib.call("cc_axpby_f64", stream(), P * P * Q * Q, 1.0, ptr(self._Gs[k]), 1.0, ptr(self._G))
+        # G[(p'p)][(q'q)] = sum_alpha A[alpha][(p'p)] B[alpha][(q'q)]
+        _lib.call("cc_gemm_f64", stream(), 1, P * P, Q * Q, na, 1.0, ptr(A._t), 0, 1, P * P, ptr(B._t), 0, Q * Q, 1,
+                  0.0, ptr(self._G), 0, Q * Q, 1)
         _lib.call("cc_permute", stream(), _lib.CC_F64, ptr(self._G), ptr(self.N_env), 4, _lib.i64_array([P, Q, P, Q]),
                   _lib.i64_array([P * Q * Q, Q, Q * Q, 1]))
         self.L.copy_(self.N_env)

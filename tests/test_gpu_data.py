@@ -46,9 +46,9 @@ def test_gemm_tile_shapes_match_numpy(gpu):
 
 
 def test_gemm_batched_match_numpy(gpu):
-    """cc_gemm_f64 batched with strides between the batch entries (the normalization environment's
-    product split over four alpha ranges is 4 x 1296 x 1296 x 1024), K-contiguous and row-contiguous
-    operands, ragged edges, alpha and beta."""
+    """cc_gemm_f64 batched with strides between the batch entries (a 1296 x 1296 product's sum split
+    over four ranges of 1024 is 4 x 1296 x 1296 x 1024), K-contiguous and row-contiguous operands,
+    ragged edges, alpha and beta."""
     import torch
     from carcassonne_amd import _lib
     from carcassonne_amd.device import ptr, stream
