@@ -139,6 +139,24 @@ __device__ inline typename LPType<T>::vec8 lds8(const T* s, int row, int t, int 
   return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7);
 }
 
+// The B operand of K tile t for the slab rows of tile nq (row 16 nq + lane % 16), the same values as
+// lds8(s, 16 nq + lane % 16, t, lane / 16).  For P % 4 == 0 the address is one lane-dependent offset
+// from s plus a compile-time constant, which keeps every read within the few address steps over which
+// the compiler's LDS lowering tags accesses with their buffer (an untagged read may be taken to alias
+// the direct global->LDS loads in flight into the other buffer, and then waits for them).
+template <typename T, int P>
+__device__ inline typename LPType<T>::vec8 lds8b(const T* s, int nq, int t, int lane) {
+  if constexpr ((P % 4) == 0) {
+    typedef typename LPType<T>::vec4 V;
+    const T* p = s + ((lane & 15) * P + 4 * (lane >> 4));
+    const V a = *reinterpret_cast<const V*>(p + (16 * nq * P + 32 * t)),
+            b = *reinterpret_cast<const V*>(p + (16 * nq * P + 32 * t + 16));
+    return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7);
+  } else {
+    return lds8<T, P>(s, 16 * nq + (lane & 15), t, lane >> 4);
+  }
+}
+
 // Stage A with the vector's hi and lo parts stacked along K: stacked index k < P holds hi at p = k,
 // P <= k < 2P holds lo at p = k - P, and the vector fragments are zero past 2P.  That takes ceil(2P/32) K
 // tiles instead of two products on each of ceil(P/32) tiles (3 instead of 4 at P = 36).  It needs
@@ -260,7 +278,7 @@ __device__ __forceinline__ void lp_term_d(f32x4 (&y)[(P + 15) / 16][DP], f32x4 (
 #pragma unroll
   for (int t = 0; t < KT; ++t)
 #pragma unroll
-    for (int nq = 0; nq < NT; ++nq) bop[t][nq] = lds8<T, P>(Bn, 16 * nq + (lane & 15), t, g);
+    for (int nq = 0; nq < NT; ++nq) bop[t][nq] = lds8b<T, P>(Bn, nq, t, lane);
   lp_combine_diag<NT, DP>(y, zp, cpd);
 #pragma unroll
   for (int nq = 0; nq < NT; ++nq)
@@ -298,7 +316,7 @@ __device__ __attribute__((always_inline)) inline void lp_alpha(const LPMeta<DP>&
 #pragma unroll
     for (int t = 0; t < KT; ++t)
 #pragma unroll
-      for (int nq = 0; nq < NT; ++nq) bop[t][nq] = lds8<T, P>(Bs, 16 * nq + (lane & 15), t, g);
+      for (int nq = 0; nq < NT; ++nq) bop[t][nq] = lds8b<T, P>(Bs, nq, t, lane);
   }
   // A operands of the next run, likewise loaded during the current run's terms
   V aop[KA];
@@ -398,7 +416,7 @@ __device__ __attribute__((always_inline)) inline void lp_alpha(const LPMeta<DP>&
 #pragma unroll
           for (int t = 0; t < KT; ++t)
 #pragma unroll
-            for (int nq = 0; nq < NT; ++nq) bop[t][nq] = lds8<T, P>(Bs, 16 * nq + (lane & 15), t, g);
+            for (int nq = 0; nq < NT; ++nq) bop[t][nq] = lds8b<T, P>(Bs, nq, t, lane);
         }
         lp_combine<NT, DP>(y, zp, cp);  // the previous term, independent of the products above
 #pragma unroll
