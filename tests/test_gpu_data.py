@@ -53,7 +53,9 @@ def test_gemm_batched_match_numpy(gpu):
     from carcassonne_amd import _lib
     from carcassonne_amd.device import ptr, stream
     rng = _rng(5)
-    for batch, M, N, K, a_t, b_t in ((4, 1296, 1296, 1024, True, False), (3, 1300, 1290, 600, False, True)):
+    # the last two: single products of one to two 64 x 64 tiles per CU, the second with ragged edges
+    for batch, M, N, K, a_t, b_t in ((4, 1296, 1296, 1024, True, False), (3, 1300, 1290, 600, False, True),
+                                     (1, 1296, 1296, 700, True, False), (1, 1100, 1000, 300, False, True)):
         a = rng.standard_normal((batch, K, M) if a_t else (batch, M, K))
         b = rng.standard_normal((batch, N, K) if b_t else (batch, K, N))
         c0 = rng.standard_normal((batch, M, N))
