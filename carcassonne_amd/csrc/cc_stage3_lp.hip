@@ -548,17 +548,22 @@ __global__ void __launch_bounds__(64 * ((P + 15) / 16), 2) lp_kernel(const LPDev
   if constexpr (DIAGF) lp_combine_diag<NT, DP>(y, zp, cpd);  // the last pending term
   else lp_combine<NT, DP>(y, zp, cp);
   // rows p' = 16 np + 4(lane/16) + r, columns q' = 16 nq + lane%16 of this workgroup's partial
+  // (d = 2: both physical components of an element in one 8-byte store)
   float* out = partial + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * NY;
 #pragma unroll
   for (int nq = 0; nq < NT; ++nq)
 #pragma unroll
-    for (int sp = 0; sp < DP; ++sp)
+    for (int r = 0; r < 4; ++r) {
+      const int pp = 16 * np + 4 * (lane >> 4) + r, qq = 16 * nq + (lane & 15);
+      if (pp < P && qq < P) {
+        if constexpr (DP == 2) {
+          *reinterpret_cast<float2*>(out + (pp * P + qq) * 2) = make_float2(y[nq][0][r], y[nq][1][r]);
+        } else {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int pp = 16 * np + 4 * (lane >> 4) + r, qq = 16 * nq + (lane & 15);
-        if (pp < P && qq < P) out[(pp * P + qq) * DP + sp] = y[nq][sp][r];
+          for (int sp = 0; sp < DP; ++sp) out[(pp * P + qq) * DP + sp] = y[nq][sp][r];
+        }
       }
-
+    }
 }
 
 // y[i] = max|v| * sum over partial rows in row order (FP64).  A workgroup owns LP_RQ * VW columns; its
