@@ -439,7 +439,7 @@ __global__ void __launch_bounds__(64 * ((P + 15) / 16), 2) lp_kernel(const LPDev
   typedef typename LT::vec8 V;
   constexpr int NT = (P + 15) / 16, KT = (P + 31) / 32, MT = DP * NT, NY = P * P * DP;
   constexpr int CPS = lp_stride(P) * 2 / 16, BUF = lp_slots(P) * CPS + lp_pad(P);
-  constexpr int NTH = 64 * NT, NV = (NY + NTH - 1) / NTH;
+  constexpr int NTH = 64 * NT;
   // stage-A operand fragments in LDS, vec8 [hi | lo][m][t][lane] (one stacked set when lp_stacked)
   constexpr bool STACK = lp_stacked<P, SPLIT>();
   constexpr int KA = lp_ka<P, SPLIT>();
@@ -455,11 +455,15 @@ __global__ void __launch_bounds__(64 * ((P + 15) / 16), 2) lp_kernel(const LPDev
   const int64_t a0 = plan->n_alpha * blockIdx.x / gridDim.x, a1 = plan->n_alpha * (blockIdx.x + 1) / gridDim.x;
   // The vector (all loads in flight at once; clamped index, masked value), then the first alpha's
   // slabs, so that both latencies overlap.
-  double xv[NV];
+  // Thread t takes the DP components of the elements pq = t + NTH k (consecutive in v), so the index
+  // arithmetic of an element is shared by its components.
+  constexpr int NQ = (P * P + NTH - 1) / NTH;
+  double xv[NQ][DP];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    const int i = threadIdx.x + NTH * k;
-    xv[k] = v[i < NY ? i : NY - 1];
+  for (int k = 0; k < NQ; ++k) {
+    const int pq = threadIdx.x + NTH * k, pc = pq < P * P ? pq : P * P - 1;
+#pragma unroll
+    for (int sv = 0; sv < DP; ++sv) xv[k][sv] = v[pc * DP + sv];
   }
   if (a0 < a1) lp_stage<P, NT>(C, n_slots, a0, buf0, np, lane);
   // Past the staged slots the buffers hold zeros (operand reads run past the last slab), and so do the
@@ -473,8 +477,11 @@ __global__ void __launch_bounds__(64 * ((P + 15) / 16), 2) lp_kernel(const LPDev
   // the same scale.
   float mx = 0.f;
 #pragma unroll
-  for (int k = 0; k < NV; ++k)
-    if (threadIdx.x + NTH * k < NY) mx = fmaxf(mx, (float)fabs(xv[k]));
+  for (int k = 0; k < NQ; ++k)
+    if (threadIdx.x + NTH * k < P * P) {
+#pragma unroll
+      for (int sv = 0; sv < DP; ++sv) mx = fmaxf(mx, (float)fabs(xv[k][sv]));
+    }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
   if (lane == 0) wmax[np] = mx;
@@ -494,15 +501,20 @@ __global__ void __launch_bounds__(64 * ((P + 15) / 16), 2) lp_kernel(const LPDev
     return (((m * KA + (kx >> 5)) * 64) + (q & 15) + 16 * g) * 8 + e;
   };
 #pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    const int i = threadIdx.x + NTH * k;
-    if (i < NY) {
-      const int sv = i % DP, pq = i / DP, q = pq % P, p = pq / P;
-      const int m = sv * NT + (q >> 4);
-      const float x = (float)(xv[k] * vinv);
-      fh[frag_index(m, p, q)] = LT::cvt(x);
-      if (STACK) fh[frag_index(m, P + p, q)] = LT::cvt(x - LT::round(x));
-      else if (NF == 2) fh[FT * 8 + frag_index(m, p, q)] = LT::cvt(x - LT::round(x));
+  for (int k = 0; k < NQ; ++k) {
+    const int pq = threadIdx.x + NTH * k;
+    if (pq < P * P) {
+      const int q = pq % P, p = pq / P;
+      // fragment m = sv NT + q / 16: the component sv sits NT * KA * 64 * 8 elements further on
+      const int f0 = frag_index(q >> 4, p, q), f1 = STACK ? frag_index(q >> 4, P + p, q) : f0;
+#pragma unroll
+      for (int sv = 0; sv < DP; ++sv) {
+        const int so = sv * NT * KA * 64 * 8;
+        const float x = (float)(xv[k][sv] * vinv);
+        fh[f0 + so] = LT::cvt(x);
+        if (STACK) fh[f1 + so] = LT::cvt(x - LT::round(x));
+        else if (NF == 2) fh[FT * 8 + f0 + so] = LT::cvt(x - LT::round(x));
+      }
     }
   }
   __syncthreads();
