@@ -452,7 +452,20 @@ __global__ void __launch_bounds__(64 * ((P + 15) / 16), 2) lp_kernel(const LPDev
   const int np = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const LPChunk* C = &plan->chunk[blockIdx.y];
   const int n_slots = C->n_slots, n_runs = C->n_runs;
-  const int64_t a0 = plan->n_alpha * blockIdx.x / gridDim.x, a1 = plan->n_alpha * (blockIdx.x + 1) / gridDim.x;
+  // this workgroup's alpha range [a0, a1): 32-bit divisions whenever n_alpha * groups fits (the 64-bit
+  // division is a long scalar sequence), with the same quotients
+  int64_t a0, a1;
+  {
+    const uint64_t na = (uint64_t)plan->n_alpha;
+    if (na * gridDim.x <= 0xFFFFFFFFull) {
+      const uint32_t n32 = (uint32_t)na;
+      a0 = (int64_t)((n32 * blockIdx.x) / gridDim.x);
+      a1 = (int64_t)((n32 * (blockIdx.x + 1)) / gridDim.x);
+    } else {
+      a0 = (int64_t)(na * blockIdx.x / gridDim.x);
+      a1 = (int64_t)(na * (blockIdx.x + 1) / gridDim.x);
+    }
+  }
   // The vector (all loads in flight at once; clamped index, masked value), then the first alpha's
   // slabs, so that both latencies overlap.
   // Thread t takes the DP components of the elements pq = t + NTH k (consecutive in v), so the index
